@@ -1,0 +1,39 @@
+// kernels.h - launch arguments and launcher entry points of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pt_device.h"
+
+namespace ptd {
+
+// entries of the per-thread traversal stack kept in LDS by the megakernel and
+// the query kernels; deeper entries go to the global overflow area
+constexpr uint32_t MEGA_LDS_DEPTH = 16;
+
+// One tile of one image (path_tracer.rs:45-57 plus the tile size and the
+// number of consecutive samples to trace in this launch).
+struct TileArgs {
+    uint32_t image_w, image_h;
+    uint32_t origin_x, origin_y;
+    uint32_t tile_w, tile_h;
+    uint32_t max_bounces;
+    uint32_t first_sample;  // 1-based sample index of the first sample
+    uint32_t samples;       // consecutive samples per pixel in this launch
+};
+
+}  // namespace ptd
+
+namespace ptcore {
+
+hipError_t launch_megakernel(const ptd::DevScene& s, const ptd::TileArgs& a, float4* accum, float4* storage,
+                             uint32_t* ovf, uint32_t grid, unsigned long long* counters, hipStream_t stream);
+hipError_t launch_primary_hits(const ptd::DevScene& s, const ptd::TileArgs& a, int32_t* tri, float* t, uint32_t* ovf,
+                               uint32_t grid, hipStream_t stream);
+hipError_t launch_intersect(const ptd::DevScene& s, const float* rays, uint32_t count, int32_t* hit, float* t, float* v,
+                            float* w, uint32_t* pairs, uint32_t* tris, uint32_t* ovf, uint32_t grid, hipStream_t stream);
+hipError_t launch_occluded(const ptd::DevScene& s, const float* rays, const float* tmax, const int32_t* skip,
+                           uint32_t count, uint8_t* out, uint32_t* ovf, uint32_t grid, hipStream_t stream);
+hipError_t launch_display(const float4* storage, uint32_t n, uchar4* out, hipStream_t stream);
+
+}  // namespace ptcore

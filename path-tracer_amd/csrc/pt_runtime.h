@@ -1,0 +1,58 @@
+// pt_runtime.h - device-side runtime objects behind the C ABI handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.h"
+#include "pt_internal.h"
+
+namespace ptcore {
+
+pt_status hip_error(hipError_t e, const char* what);
+
+// One HIP device and stream (the reference's wgpu device + queue).
+struct Context {
+    int device = 0;
+    uint32_t cu_count = 0;
+    char arch[64] = {0};
+    hipStream_t stream = nullptr;
+    float* srgb_lut = nullptr;            // 256-entry sRGB decode table
+    uint32_t* ovf = nullptr;              // grid-strided traversal overflow stacks
+    size_t ovf_bytes = 0;
+    unsigned long long* counters = nullptr;  // device counters of the last profiled launch
+    // make room for `depth` stack entries for `grid_threads` threads
+    pt_status ensure_overflow(uint32_t depth, uint32_t grid_threads);
+};
+
+// A scene resident in device memory (WorldResources).
+struct Scene {
+    Context* ctx = nullptr;
+    ptd::DevScene view{};
+    uint32_t bvh_depth = 0;
+    uint32_t triangle_count = 0, material_count = 0, bvh_node_count = 0;
+    uint32_t scene_alias_capacity = 0, skybox_alias_count = 0;
+    uint64_t footprint = 0;
+    uint64_t version = 0;  // bumped by every update; tracers reset accumulation on change
+};
+
+// RAII device allocation for temporaries of one call.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    pt_status alloc(size_t n) {
+        hipError_t e = hipMalloc(&ptr, n ? n : 1);
+        if (e != hipSuccess) return hip_error(e, "temporary device buffer");
+        bytes = n;
+        return PT_OK;
+    }
+    ~DeviceBuffer() {
+        if (ptr) hipFree(ptr);
+    }
+};
+
+}  // namespace ptcore
+
+struct pt_context : ptcore::Context {};
+struct pt_scene : ptcore::Scene {};

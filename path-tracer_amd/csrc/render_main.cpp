@@ -1,0 +1,257 @@
+// render_main.cpp - pt-render: the reference's `render` command on libptcore.
+//
+// Keeps the reference's flag set and semantics (src/render/headless.rs:14-127):
+//   --scene --skybox-luminance --output --width --height --samples
+//   --max-bounces --tile-size --camera-position X,Y,Z --camera-yaw
+//   --camera-pitch --camera-fov
+// and its tile loop (headless.rs:207-285): the image is traced tile by tile,
+// each tile to all of its samples, at most 16 samples per trace call, then
+// tone-mapped (display.wgsl) and written as RGB8. Scenes are the procedural
+// generators (cornell | field | large | sized:<triangles>) because .glb
+// ingest is not part of this core. Extra flags: --seed, --device,
+// --schedule mega|wavefront, --pfm <file> (linear storage image dump).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pt_api.h"
+#include "../../include/pt_scenes.h"
+
+namespace {
+
+[[noreturn]] void die(const char* fmt, const char* a = "") {
+    std::fprintf(stderr, "error: ");
+    std::fprintf(stderr, fmt, a);
+    std::fprintf(stderr, "\n");
+    std::exit(1);
+}
+void check(pt_status s, const char* what) {
+    if (s != PT_OK) {
+        std::fprintf(stderr, "error: %s: %s\n", what, pt_last_error());
+        std::exit(1);
+    }
+}
+float parse_finite(const char* v, const char* flag) {
+    char* end = nullptr;
+    float f = std::strtof(v, &end);
+    if (!end || *end || !std::isfinite(f)) die("%s must be a finite number", flag);
+    return f;
+}
+uint32_t parse_u32(const char* v, const char* flag, uint32_t lo, uint32_t hi) {
+    char* end = nullptr;
+    unsigned long x = std::strtoul(v, &end, 10);
+    if (!end || *end || x < lo || x > hi) die("%s out of range", flag);
+    return (uint32_t)x;
+}
+
+uint32_t crc32(const uint8_t* p, size_t n, uint32_t c = 0xffffffffu) {
+    static uint32_t table[256];
+    static bool init = false;
+    if (!init) {
+        for (uint32_t i = 0; i < 256; ++i) {
+            uint32_t k = i;
+            for (int j = 0; j < 8; ++j) k = (k & 1) ? 0xedb88320u ^ (k >> 1) : k >> 1;
+            table[i] = k;
+        }
+        init = true;
+    }
+    for (size_t i = 0; i < n; ++i) c = table[(c ^ p[i]) & 255u] ^ (c >> 8);
+    return c;
+}
+
+// PNG with zlib "stored" (uncompressed) deflate blocks: RGB8, no filtering
+void write_png(const std::string& path, const std::vector<uint8_t>& rgb, uint32_t w, uint32_t h) {
+    std::vector<uint8_t> raw;
+    raw.reserve((size_t)(w * 3 + 1) * h);
+    for (uint32_t y = 0; y < h; ++y) {
+        raw.push_back(0);
+        raw.insert(raw.end(), rgb.begin() + (size_t)y * w * 3, rgb.begin() + (size_t)(y + 1) * w * 3);
+    }
+    std::vector<uint8_t> z = {0x78, 0x01};
+    uint32_t a = 1, b = 0;
+    for (uint8_t c : raw) {
+        a = (a + c) % 65521u;
+        b = (b + a) % 65521u;
+    }
+    for (size_t off = 0; off < raw.size() || off == 0; off += 65535) {
+        size_t len = std::min<size_t>(65535, raw.size() - off);
+        z.push_back(off + len >= raw.size() ? 1 : 0);
+        z.push_back(len & 255);
+        z.push_back(len >> 8);
+        z.push_back(~len & 255);
+        z.push_back((~len >> 8) & 255);
+        z.insert(z.end(), raw.begin() + off, raw.begin() + off + len);
+        if (len == 0) break;
+    }
+    uint32_t adler = (b << 16) | a;
+    for (int k = 3; k >= 0; --k) z.push_back((adler >> (8 * k)) & 255);
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) die("cannot write %s", path.c_str());
+    const uint8_t sig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
+    std::fwrite(sig, 1, 8, f);
+    auto chunk = [&](const char* type, const std::vector<uint8_t>& data) {
+        uint8_t len[4] = {(uint8_t)(data.size() >> 24), (uint8_t)(data.size() >> 16), (uint8_t)(data.size() >> 8),
+                          (uint8_t)data.size()};
+        std::fwrite(len, 1, 4, f);
+        std::vector<uint8_t> td(type, type + 4);
+        td.insert(td.end(), data.begin(), data.end());
+        std::fwrite(td.data(), 1, td.size(), f);
+        uint32_t c = crc32(td.data(), td.size()) ^ 0xffffffffu;
+        uint8_t cb[4] = {(uint8_t)(c >> 24), (uint8_t)(c >> 16), (uint8_t)(c >> 8), (uint8_t)c};
+        std::fwrite(cb, 1, 4, f);
+    };
+    std::vector<uint8_t> ihdr = {(uint8_t)(w >> 24), (uint8_t)(w >> 16), (uint8_t)(w >> 8), (uint8_t)w,
+                                 (uint8_t)(h >> 24), (uint8_t)(h >> 16), (uint8_t)(h >> 8), (uint8_t)h,
+                                 8, 2, 0, 0, 0};
+    chunk("IHDR", ihdr);
+    chunk("IDAT", z);
+    chunk("IEND", {});
+    std::fclose(f);
+}
+
+void write_pfm(const std::string& path, const std::vector<float>& rgb, uint32_t w, uint32_t h) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) die("cannot write %s", path.c_str());
+    std::fprintf(f, "PF\n%u %u\n-1.0\n", w, h);
+    for (int64_t y = (int64_t)h - 1; y >= 0; --y) std::fwrite(&rgb[(size_t)y * w * 3], 4, (size_t)w * 3, f);
+    std::fclose(f);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::string scene = "cornell", output = "render.png", pfm, schedule = "wavefront";
+    uint32_t width = 1920, height = 1080, samples = 256, max_bounces = PT_DEFAULT_MAX_BOUNCES, tile = 512, device = 0;
+    uint64_t seed = 1;
+    float sky_lum = -1.0f;
+    float cam_pos[3] = {NAN, NAN, NAN};
+    float yaw = NAN, pitch = NAN, fov = NAN;
+    int first = 1;
+    if (argc > 1 && std::strcmp(argv[1], "render") == 0) first = 2;
+    for (int i = first; i < argc; ++i) {
+        std::string a = argv[i];
+        auto val = [&]() -> const char* {
+            if (i + 1 >= argc) die("%s needs a value", a.c_str());
+            return argv[++i];
+        };
+        if (a == "--scene") scene = val();
+        else if (a == "--output") output = val();
+        else if (a == "--pfm") pfm = val();
+        else if (a == "--width") width = parse_u32(val(), "--width", 1, 1u << 16);
+        else if (a == "--height") height = parse_u32(val(), "--height", 1, 1u << 16);
+        else if (a == "--samples") samples = parse_u32(val(), "--samples", 1, 1u << 24);
+        else if (a == "--max-bounces") max_bounces = parse_u32(val(), "--max-bounces", 0, PT_MAX_BOUNCES_LIMIT);
+        else if (a == "--tile-size") tile = parse_u32(val(), "--tile-size", 1, 1u << 16);
+        else if (a == "--seed") seed = std::strtoull(val(), nullptr, 10);
+        else if (a == "--device") device = parse_u32(val(), "--device", 0, 64);
+        else if (a == "--schedule") schedule = val();
+        else if (a == "--skybox-luminance") {
+            sky_lum = parse_finite(val(), "--skybox-luminance");
+            if (sky_lum < 0.0f) die("--skybox-luminance must be nonnegative");
+        } else if (a == "--camera-position") {
+            std::string v = val();
+            float xyz[3];
+            size_t p = 0;
+            for (int k = 0; k < 3; ++k) {
+                size_t q = v.find(',', p);
+                if ((k < 2) != (q != std::string::npos)) die("--camera-position must be three comma-separated coordinates");
+                xyz[k] = parse_finite(v.substr(p, q == std::string::npos ? std::string::npos : q - p).c_str(), "--camera-position");
+                p = q + 1;
+            }
+            for (int k = 0; k < 3; ++k) cam_pos[k] = xyz[k];
+        } else if (a == "--camera-yaw") yaw = parse_finite(val(), "--camera-yaw");
+        else if (a == "--camera-pitch") {
+            pitch = parse_finite(val(), "--camera-pitch");
+            if (pitch < -90.0f || pitch > 90.0f) die("--camera-pitch must be from -90 to 90");
+        } else if (a == "--camera-fov") {
+            fov = parse_finite(val(), "--camera-fov");
+            if (!(fov > 0.0f && fov < 180.0f)) die("--camera-fov must be strictly between 0 and 180");
+        } else if (a == "--help" || a == "-h") {
+            std::printf("usage: pt-render [render] --scene cornell|field|large|sized:N [--width W] [--height H]\n"
+                        "  [--samples S] [--max-bounces D] [--tile-size T] [--output out.png] [--pfm out.pfm]\n"
+                        "  [--skybox-luminance L] [--camera-position X,Y,Z] [--camera-yaw DEG]\n"
+                        "  [--camera-pitch DEG] [--camera-fov DEG] [--seed N] [--schedule mega|wavefront]\n");
+            return 0;
+        } else die("unknown argument %s", a.c_str());
+    }
+    const auto started = std::chrono::steady_clock::now();
+    int kind = 0;
+    uint32_t tris = 0;
+    if (scene == "cornell") kind = PT_SCENE_CORNELL;
+    else if (scene == "field") kind = PT_SCENE_FIELD;
+    else if (scene == "large") kind = PT_SCENE_LARGE;
+    else if (scene.rfind("sized:", 0) == 0) {
+        kind = PT_SCENE_SIZED;
+        tris = parse_u32(scene.c_str() + 6, "--scene sized:N", 64, 1u << 30);
+    } else die("unknown scene %s (cornell, field, large, sized:N)", scene.c_str());
+
+    pt_generated* gen = nullptr;
+    check(pt_generate_scene(kind, seed, tris, width, height, &gen), "generate scene");
+    pt_world_desc wd;
+    check(pt_generated_world_desc(gen, &wd), "scene description");
+    if (sky_lum >= 0.0f) wd.skybox_luminance = sky_lum;
+    if (!std::isnan(cam_pos[0]))
+        for (int k = 0; k < 3; ++k) wd.camera_position[k] = cam_pos[k];
+    if (!std::isnan(yaw)) wd.camera_yaw_degrees = yaw;
+    if (!std::isnan(pitch)) wd.camera_pitch_degrees = pitch;
+    if (!std::isnan(fov)) wd.camera_fov_degrees = fov;
+    wd.aspect_ratio = (float)width / (float)height;
+    pt_world* world = nullptr;
+    check(pt_world_build(&wd, &world), "build world");
+    pt_scene_desc sd;
+    check(pt_world_scene_desc(world, &sd), "scene arrays");
+    std::printf("Loaded scene %s with %u triangles, %u materials, %u BVH nodes\n", scene.c_str(), sd.triangle_count,
+                sd.material_count, sd.bvh_node_count);
+
+    pt_context* ctx = nullptr;
+    check(pt_context_create((int)device, &ctx), "create context");
+    pt_scene* dscene = nullptr;
+    check(pt_scene_create(ctx, &sd, &dscene), "upload scene");
+    pt_world_destroy(world);  // the GPU has its own copy (headless.rs:157)
+    pt_generated_destroy(gen);
+
+    const uint32_t tw = std::min(tile, width), th = std::min(tile, height);
+    pt_tracer* tracer = nullptr;
+    check(pt_tracer_create(ctx, dscene, tw, th, &tracer), "create tracer");
+    check(pt_tracer_set_schedule(tracer, schedule == "mega" ? PT_SCHEDULE_MEGAKERNEL : PT_SCHEDULE_WAVEFRONT), "schedule");
+    check(pt_tracer_set_max_bounces(tracer, max_bounces), "max bounces");
+    std::vector<uint8_t> rgb((size_t)width * height * 3);
+    std::vector<float> lin(pfm.empty() ? 0 : (size_t)width * height * 3);
+    std::vector<uint8_t> tile_rgba((size_t)tw * th * 4);
+    std::vector<float> tile_f(pfm.empty() ? 0 : (size_t)tw * th * 4);
+    const auto trace_start = std::chrono::steady_clock::now();
+    for (uint32_t y = 0; y < height; y += th) {
+        for (uint32_t x = 0; x < width; x += tw) {
+            const uint32_t vw = std::min(tw, width - x), vh = std::min(th, height - y);
+            check(pt_tracer_set_tile(tracer, width, height, x, y, tw, th), "set tile");
+            uint32_t remaining = samples;
+            while (remaining > 0) {
+                uint32_t n = 0;
+                check(pt_tracer_trace(tracer, remaining, &n), "trace");
+                remaining -= n;
+            }
+            check(pt_tracer_display_rgba8(tracer, tile_rgba.data()), "display");
+            if (!pfm.empty()) check(pt_tracer_read_storage(tracer, tile_f.data()), "read storage");
+            for (uint32_t r = 0; r < vh; ++r)
+                for (uint32_t c = 0; c < vw; ++c)
+                    for (int k = 0; k < 3; ++k) {
+                        rgb[((size_t)(y + r) * width + x + c) * 3 + k] = tile_rgba[((size_t)r * tw + c) * 4 + k];
+                        if (!pfm.empty()) lin[((size_t)(y + r) * width + x + c) * 3 + k] = tile_f[((size_t)r * tw + c) * 4 + k];
+                    }
+        }
+    }
+    const double trace_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - trace_start).count();
+    write_png(output, rgb, width, height);
+    if (!pfm.empty()) write_pfm(pfm, lin, width, height);
+    const double total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - started).count();
+    std::printf("Saved %s in %.1fs (trace %.3fs, %.3g path samples/s)\n", output.c_str(), total_s, trace_s,
+                (double)width * height * samples / trace_s);
+    pt_tracer_destroy(tracer);
+    pt_scene_destroy(dscene);
+    pt_context_destroy(ctx);
+    return 0;
+}

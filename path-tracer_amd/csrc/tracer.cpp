@@ -1,0 +1,302 @@
+// tracer.cpp - pt_tracer: the PathTracer of the reference (path_tracer.rs:59-358).
+//
+// A tracer owns the tile's accumulation buffer (float4 per tile pixel, the
+// running sum of samples) and storage image (EXPOSURE * sum / samples), and
+// the per-sample parameters: image size, tile origin, max bounces and the
+// 1-based sample count, which set_tile / set_max_bounces / scene updates reset
+// exactly as PathTracer does (path_tracer.rs:264-313, 272-282).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "pt_runtime.h"
+#include "wavefront.h"
+
+using namespace ptcore;
+
+struct pt_tracer {
+    Context* ctx = nullptr;
+    Scene* scene = nullptr;
+    uint64_t scene_version = 0;
+    uint32_t tile_w = 0, tile_h = 0;
+    PtTraceParameters params{};
+    int schedule = PT_SCHEDULE_WAVEFRONT;
+    float4* accum = nullptr;
+    float4* storage = nullptr;
+    int profiling = 0;
+    pt_trace_stats stats{};
+    Wavefront* wave = nullptr;
+};
+
+namespace {
+
+pt_status alloc_tile(pt_tracer* t, uint32_t w, uint32_t h) {
+    hipFree(t->accum);
+    hipFree(t->storage);
+    t->accum = t->storage = nullptr;
+    size_t n = (size_t)w * h * sizeof(float4);
+    hipError_t e = hipMalloc(&t->accum, n);
+    if (e == hipSuccess) e = hipMalloc(&t->storage, n);
+    if (e == hipSuccess) e = hipMemsetAsync(t->accum, 0, n, t->ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(t->storage, 0, n, t->ctx->stream);
+    if (e != hipSuccess) return hip_error(e, "tile buffers");
+    t->tile_w = w;
+    t->tile_h = h;
+    return PT_OK;
+}
+
+// PathTracer::validate_tile_size (path_tracer.rs:361-382): nonzero, and the
+// accumulation buffer must be addressable
+pt_status validate_tile(uint32_t w, uint32_t h) {
+    if (w == 0 || h == 0) return set_error(PT_ERR_INVALID, "tile size must be nonzero");
+    if ((uint64_t)w * h > (1ull << 31)) return set_error(PT_ERR_INVALID, "tile of %ux%u is too large", w, h);
+    return PT_OK;
+}
+
+void sync_version(pt_tracer* t) {
+    if (t->scene->version != t->scene_version) {
+        t->scene_version = t->scene->version;
+        t->params.sample_count = 0;  // reset_accumulation (path_tracer.rs:278-280)
+    }
+}
+
+ptd::TileArgs tile_args(const pt_tracer* t, uint32_t first_sample, uint32_t samples) {
+    ptd::TileArgs a;
+    a.image_w = t->params.image_size[0];
+    a.image_h = t->params.image_size[1];
+    a.origin_x = t->params.tile_origin[0];
+    a.origin_y = t->params.tile_origin[1];
+    // only the part of the tile within the image is traced (path_tracer.rs:336-340)
+    a.tile_w = t->tile_w;
+    a.tile_h = t->tile_h;
+    a.max_bounces = t->params.max_bounces;
+    a.first_sample = first_sample;
+    a.samples = samples;
+    return a;
+}
+
+pt_status trace_samples(pt_tracer* t, uint32_t samples) {
+    Context* c = t->ctx;
+    hipSetDevice(c->device);
+    sync_version(t);
+    const uint32_t first = t->params.sample_count + 1u;
+    ptd::TileArgs a = tile_args(t, first, samples);
+    pt_status st;
+    if (t->schedule == PT_SCHEDULE_MEGAKERNEL) {
+        const uint32_t blocks = ((t->tile_w + 15u) / 16u) * ((t->tile_h + 15u) / 16u);
+        const uint32_t grid = std::min<uint32_t>(blocks, c->cu_count * 8u);
+        if ((st = c->ensure_overflow(t->scene->bvh_depth + 1u, grid * 256u))) return st;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (t->profiling) {
+            hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream);
+            hipEventCreate(&e0);
+            hipEventCreate(&e1);
+            hipEventRecord(e0, c->stream);
+        }
+        hipError_t e = launch_megakernel(t->scene->view, a, t->accum, t->storage, c->ovf, grid,
+                                         t->profiling ? c->counters : nullptr, c->stream);
+        if (e != hipSuccess) return hip_error(e, "megakernel launch");
+        if (t->profiling) {
+            hipEventRecord(e1, c->stream);
+            hipEventSynchronize(e1);
+            unsigned long long h[4];
+            hipMemcpy(h, c->counters, sizeof(h), hipMemcpyDeviceToHost);
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, e0, e1);
+            hipEventDestroy(e0);
+            hipEventDestroy(e1);
+            std::memset(&t->stats, 0, sizeof(t->stats));
+            t->stats.closest_rays = h[0];
+            t->stats.shadow_rays = h[1];
+            t->stats.pair_tests = h[2];
+            t->stats.tri_tests = h[3];
+            t->stats.total_ms = ms;
+        }
+    } else {
+        if (!t->wave) {
+            t->wave = new Wavefront();
+            t->wave->ctx = c;
+        }
+        if ((st = t->wave->render(*t->scene, a, t->accum, t->storage, t->profiling, &t->stats))) return st;
+    }
+    t->params.sample_count += samples;
+    return PT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pt_status pt_tracer_create(pt_context* c, pt_scene* s, uint32_t tile_w, uint32_t tile_h, pt_tracer** out) {
+    if (!c || !s || !out) return set_error(PT_ERR_INVALID, "pt_tracer_create: null argument");
+    *out = nullptr;
+    if (s->ctx != c) return set_error(PT_ERR_INVALID, "scene belongs to another context");
+    pt_status st = validate_tile(tile_w, tile_h);
+    if (st) return st;
+    hipSetDevice(c->device);
+    pt_tracer* t = new pt_tracer();
+    t->ctx = c;
+    t->scene = s;
+    t->scene_version = s->version;
+    t->params.image_size[0] = tile_w;
+    t->params.image_size[1] = tile_h;
+    t->params.max_bounces = PT_DEFAULT_MAX_BOUNCES;
+    if ((st = alloc_tile(t, tile_w, tile_h))) {
+        pt_tracer_destroy(t);
+        return st;
+    }
+    *out = t;
+    return PT_OK;
+}
+
+pt_status pt_tracer_destroy(pt_tracer* t) {
+    if (!t) return PT_OK;
+    hipSetDevice(t->ctx->device);
+    hipStreamSynchronize(t->ctx->stream);
+    delete t->wave;
+    hipFree(t->accum);
+    hipFree(t->storage);
+    delete t;
+    return PT_OK;
+}
+
+pt_status pt_tracer_set_schedule(pt_tracer* t, int schedule) {
+    if (!t) return set_error(PT_ERR_INVALID, "null tracer");
+    if (schedule != PT_SCHEDULE_MEGAKERNEL && schedule != PT_SCHEDULE_WAVEFRONT)
+        return set_error(PT_ERR_INVALID, "unknown schedule %d", schedule);
+    t->schedule = schedule;
+    return PT_OK;
+}
+
+pt_status pt_tracer_set_tile(pt_tracer* t, uint32_t iw, uint32_t ih, uint32_t ox, uint32_t oy, uint32_t tw, uint32_t th) {
+    if (!t) return set_error(PT_ERR_INVALID, "null tracer");
+    if (iw == 0 || ih == 0) return set_error(PT_ERR_INVALID, "image size must be nonzero");
+    if (ox >= iw || oy >= ih) return set_error(PT_ERR_INVALID, "tile origin must be within the image");
+    pt_status st = validate_tile(tw, th);
+    if (st) return st;
+    hipSetDevice(t->ctx->device);
+    if (tw != t->tile_w || th != t->tile_h) {
+        hipStreamSynchronize(t->ctx->stream);
+        if ((st = alloc_tile(t, tw, th))) return st;
+    }
+    t->params.image_size[0] = iw;
+    t->params.image_size[1] = ih;
+    t->params.tile_origin[0] = ox;
+    t->params.tile_origin[1] = oy;
+    t->params.sample_count = 0;
+    t->scene_version = t->scene->version;
+    return PT_OK;
+}
+
+pt_status pt_tracer_set_max_bounces(pt_tracer* t, uint32_t mb) {
+    if (!t) return set_error(PT_ERR_INVALID, "null tracer");
+    if (mb > PT_MAX_BOUNCES_LIMIT) return set_error(PT_ERR_INVALID, "max bounces %u above %u", mb, PT_MAX_BOUNCES_LIMIT);
+    if (t->params.max_bounces != mb) {
+        t->params.max_bounces = mb;
+        t->params.sample_count = 0;
+    }
+    return PT_OK;
+}
+
+pt_status pt_tracer_set_sample_count(pt_tracer* t, uint32_t n) {
+    if (!t) return set_error(PT_ERR_INVALID, "null tracer");
+    sync_version(t);
+    t->params.sample_count = n;
+    return PT_OK;
+}
+
+uint32_t pt_tracer_sample_count(const pt_tracer* t) { return t ? t->params.sample_count : 0; }
+uint32_t pt_tracer_max_bounces(const pt_tracer* t) { return t ? t->params.max_bounces : 0; }
+
+pt_status pt_tracer_trace(pt_tracer* t, uint32_t samples, uint32_t* traced) {
+    if (!t) return set_error(PT_ERR_INVALID, "null tracer");
+    uint32_t n = std::min(samples, PT_MAX_SAMPLES_PER_TRACE);  // path_tracer.rs:322
+    if (traced) *traced = 0;
+    if (n == 0) return PT_OK;
+    pt_status st = trace_samples(t, n);
+    if (st == PT_OK && traced) *traced = n;
+    return st;
+}
+
+pt_status pt_tracer_trace_all(pt_tracer* t, uint32_t samples) {
+    if (!t) return set_error(PT_ERR_INVALID, "null tracer");
+    if (samples == 0) return PT_OK;
+    return trace_samples(t, samples);
+}
+
+pt_status pt_tracer_read_accumulation(pt_tracer* t, float* rgba) {
+    if (!t || !rgba) return set_error(PT_ERR_INVALID, "null argument");
+    hipSetDevice(t->ctx->device);
+    hipError_t e = hipMemcpyAsync(rgba, t->accum, (size_t)t->tile_w * t->tile_h * sizeof(float4), hipMemcpyDeviceToHost,
+                                  t->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->ctx->stream);
+    return e == hipSuccess ? PT_OK : hip_error(e, "read accumulation");
+}
+
+pt_status pt_tracer_write_accumulation(pt_tracer* t, const float* rgba) {
+    if (!t || !rgba) return set_error(PT_ERR_INVALID, "null argument");
+    hipSetDevice(t->ctx->device);
+    hipError_t e = hipMemcpyAsync(t->accum, rgba, (size_t)t->tile_w * t->tile_h * sizeof(float4), hipMemcpyHostToDevice,
+                                  t->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->ctx->stream);
+    return e == hipSuccess ? PT_OK : hip_error(e, "write accumulation");
+}
+
+pt_status pt_tracer_read_storage(pt_tracer* t, float* rgba) {
+    if (!t || !rgba) return set_error(PT_ERR_INVALID, "null argument");
+    hipSetDevice(t->ctx->device);
+    hipError_t e = hipMemcpyAsync(rgba, t->storage, (size_t)t->tile_w * t->tile_h * sizeof(float4), hipMemcpyDeviceToHost,
+                                  t->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->ctx->stream);
+    return e == hipSuccess ? PT_OK : hip_error(e, "read storage");
+}
+
+pt_status pt_tracer_display_rgba8(pt_tracer* t, uint8_t* rgba) {
+    if (!t || !rgba) return set_error(PT_ERR_INVALID, "null argument");
+    hipSetDevice(t->ctx->device);
+    const uint32_t n = t->tile_w * t->tile_h;
+    DeviceBuffer buf;
+    pt_status st = buf.alloc((size_t)n * 4);
+    if (st) return st;
+    hipError_t e = launch_display(t->storage, n, (uchar4*)buf.ptr, t->ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(rgba, buf.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, t->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->ctx->stream);
+    return e == hipSuccess ? PT_OK : hip_error(e, "display");
+}
+
+pt_status pt_tracer_primary_hits(pt_tracer* t, uint32_t sample_count, int32_t* tri, float* tt) {
+    if (!t || !tri || !tt) return set_error(PT_ERR_INVALID, "null argument");
+    if (sample_count == 0) return set_error(PT_ERR_INVALID, "sample_count is 1-based");
+    Context* c = t->ctx;
+    hipSetDevice(c->device);
+    const uint32_t n = t->tile_w * t->tile_h;
+    const uint32_t blocks = ((t->tile_w + 15u) / 16u) * ((t->tile_h + 15u) / 16u);
+    const uint32_t grid = std::min<uint32_t>(blocks, c->cu_count * 8u);
+    pt_status st = c->ensure_overflow(t->scene->bvh_depth + 1u, grid * 256u);
+    if (st) return st;
+    DeviceBuffer buf;
+    if ((st = buf.alloc((size_t)n * 8))) return st;
+    int32_t* dtri = (int32_t*)buf.ptr;
+    float* dt = (float*)(dtri + n);
+    ptd::TileArgs a = tile_args(t, sample_count, 1);
+    hipError_t e = launch_primary_hits(t->scene->view, a, dtri, dt, c->ovf, grid, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(tri, dtri, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(tt, dt, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? PT_OK : hip_error(e, "primary hits");
+}
+
+pt_status pt_tracer_set_profiling(pt_tracer* t, int enabled) {
+    if (!t) return set_error(PT_ERR_INVALID, "null tracer");
+    t->profiling = enabled ? 1 : 0;
+    return PT_OK;
+}
+
+pt_status pt_tracer_stats(pt_tracer* t, pt_trace_stats* out) {
+    if (!t || !out) return set_error(PT_ERR_INVALID, "null argument");
+    *out = t->stats;
+    return PT_OK;
+}
+
+}  // extern "C"

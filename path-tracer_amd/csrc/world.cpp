@@ -1,0 +1,473 @@
+// world.cpp - host scene preparation: the producers of the kernel's inputs.
+//
+// Restates, in C++, what the reference's World does between loading a scene
+// and uploading it (src/render/world.rs):
+//   * binned-SAH BVH with 8 bins over centroid bounds, leaf at <= 2 triangles
+//     or when splitting costs more than the leaf, Hoare partition, children
+//     allocated as an adjacent pair, recursion left then right
+//     (world.rs:1339-1345, 1430-1591);
+//   * triangle reorder into BVH order (world.rs:1348-1355);
+//   * Vose alias tables in f64 (world.rs:539-575), emissive-triangle weights
+//     2*pi*area*luminance (world.rs:1691-1735), sky weights per row solid
+//     angle (world.rs:584-587, 1959-1975), fluxes and category probabilities
+//     (world.rs:484-498, 1977-1987, 1738-1741);
+//   * sky normalisation by solid-angle-weighted mean luminance (world.rs:1872-1957);
+//   * camera (world.rs:2159-2193) and the headless option handling
+//     (headless.rs:180-204).
+// Floating-point steps follow the reference's types (f32 with mul_add where
+// it uses mul_add, f64 where it widens), so the arrays come out bit-identical.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pt_internal.h"
+
+namespace {
+
+struct Aabb {
+    float mn[3] = {INFINITY, INFINITY, INFINITY};
+    float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const Aabb& o) {
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = std::fmin(mn[k], o.mn[k]);
+            mx[k] = std::fmax(mx[k], o.mx[k]);
+        }
+    }
+};
+
+// e.x*e.y + e.y*e.z + e.z*e.x with the reference's mul_add nesting (world.rs:412-416)
+inline float half_area(const float mn[3], const float mx[3]) {
+    float ex = mx[0] - mn[0], ey = mx[1] - mn[1], ez = mx[2] - mn[2];
+    return std::fmaf(ez, ex, std::fmaf(ex, ey, ey * ez));
+}
+
+struct BvhTri {
+    Aabb box;
+    float centroid[3];
+    uint32_t original;
+};
+
+// Rust `f32 as usize`: saturating, NaN -> 0
+inline size_t f32_as_usize(float v) {
+    if (!(v > 0.0f)) return 0;
+    if (v >= 1.8446744e19f) return std::numeric_limits<size_t>::max();
+    return (size_t)v;
+}
+
+struct Builder {
+    std::vector<PtBvhNode>& nodes;
+    std::vector<BvhTri>& tris;
+
+    void set_box(size_t ni, size_t start, size_t end) {
+        Aabb b;
+        for (size_t i = start; i < end; ++i) b.grow(tris[i].box);
+        for (int k = 0; k < 3; ++k) {
+            nodes[ni].aabb_min[k] = b.mn[k];
+            nodes[ni].aabb_max[k] = b.mx[k];
+        }
+    }
+
+    struct Item {
+        size_t node, start, end;
+    };
+
+    // world.rs:1430-1591, iterative over an explicit stack that visits the
+    // left child's subtree before the right one, as the recursion does, so
+    // nodes are allocated in the same order
+    void subdivide_all(size_t total) {
+        std::vector<Item> stack;
+        stack.push_back({0, 0, total});
+        while (!stack.empty()) {
+            Item it = stack.back();
+            stack.pop_back();
+            Item left{}, right{};
+            if (!subdivide(it.node, it.start, it.end, left, right)) continue;
+            stack.push_back(right);
+            stack.push_back(left);
+        }
+    }
+
+    // split one node; returns false if it became a leaf
+    bool subdivide(size_t ni, size_t start, size_t end, Item& left_out, Item& right_out) {
+        const size_t tri_count = end - start;
+        if (tri_count <= 2) {
+            nodes[ni].left_first = (uint32_t)start;
+            nodes[ni].tri_count = (uint32_t)tri_count;
+            return false;
+        }
+        float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (size_t i = start; i < end; ++i) {
+            for (int k = 0; k < 3; ++k) {
+                cmin[k] = std::fmin(cmin[k], tris[i].centroid[k]);
+                cmax[k] = std::fmax(cmax[k], tris[i].centroid[k]);
+            }
+        }
+        int best_axis = 0;
+        size_t best_split = 0;
+        float best_cost = std::numeric_limits<float>::max();
+        for (int axis = 0; axis < 3; ++axis) {
+            const float amin = cmin[axis], amax = cmax[axis];
+            if (amin == amax) continue;
+            Aabb bins[PT_BVH_BINS];
+            uint32_t counts[PT_BVH_BINS] = {0};
+            const float scale = (float)PT_BVH_BINS / (amax - amin);
+            for (size_t i = start; i < end; ++i) {
+                size_t b = std::min(f32_as_usize((tris[i].centroid[axis] - amin) * scale), (size_t)PT_BVH_BINS - 1);
+                counts[b] += 1;
+                bins[b].grow(tris[i].box);
+            }
+            float left_area[PT_BVH_BINS - 1], right_area[PT_BVH_BINS - 1];
+            uint32_t left_count[PT_BVH_BINS - 1], right_count[PT_BVH_BINS - 1];
+            Aabb lb;
+            uint32_t ls = 0;
+            for (int i = 0; i < PT_BVH_BINS - 1; ++i) {
+                ls += counts[i];
+                lb.grow(bins[i]);
+                left_count[i] = ls;
+                left_area[i] = half_area(lb.mn, lb.mx);
+            }
+            Aabb rb;
+            uint32_t rs = 0;
+            for (int i = PT_BVH_BINS - 1; i >= 1; --i) {
+                rs += counts[i];
+                rb.grow(bins[i]);
+                right_count[i - 1] = rs;
+                right_area[i - 1] = half_area(rb.mn, rb.mx);
+            }
+            for (int i = 0; i < PT_BVH_BINS - 1; ++i) {
+                float cost = std::fmaf((float)left_count[i], left_area[i], (float)right_count[i] * right_area[i]);
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_axis = axis;
+                    best_split = (size_t)i;
+                }
+            }
+        }
+        const float leaf_cost = (float)tri_count * half_area(nodes[ni].aabb_min, nodes[ni].aabb_max);
+        if (best_cost >= leaf_cost) {
+            nodes[ni].left_first = (uint32_t)start;
+            nodes[ni].tri_count = (uint32_t)tri_count;
+            return false;
+        }
+        const float amin = cmin[best_axis], amax = cmax[best_axis];
+        const float scale = (float)PT_BVH_BINS / (amax - amin);
+        auto get_bin = [&](float c) { return std::min(f32_as_usize((c - amin) * scale), (size_t)PT_BVH_BINS - 1); };
+        // modified Hoare partition (world.rs:1554-1573)
+        int64_t left = (int64_t)start, right = (int64_t)end - 1;
+        for (;;) {
+            while (left <= right && get_bin(tris[left].centroid[best_axis]) <= best_split) ++left;
+            while (right > left && get_bin(tris[right].centroid[best_axis]) > best_split) --right;
+            if (left >= right) break;
+            std::swap(tris[left], tris[right]);
+            ++left;
+            --right;
+        }
+        const size_t split = (size_t)left;
+        const size_t lc = nodes.size(), rc = lc + 1;
+        nodes.push_back(PtBvhNode{});
+        nodes.push_back(PtBvhNode{});
+        set_box(lc, start, split);
+        set_box(rc, split, end);
+        nodes[ni].left_first = (uint32_t)lc;
+        nodes[ni].tri_count = 0;
+        left_out = {lc, start, split};
+        right_out = {rc, split, end};
+        return true;
+    }
+};
+
+// luminance of max(rgb, 0) in f64 (world.rs:578-581)
+inline double luminance(const float* rgb) {
+    double r = std::fmax((double)rgb[0], 0.0), g = std::fmax((double)rgb[1], 0.0), b = std::fmax((double)rgb[2], 0.0);
+    return (r * 0.2126 + g * 0.7152) + b * 0.0722;
+}
+// exact equirectangular row solid angle without the longitude factor (world.rs:584-587)
+inline double row_weight(uint32_t y, uint32_t height) {
+    const double step = M_PI / (double)height;
+    return std::cos((double)y * step) - std::cos((double)(y + 1) * step);
+}
+
+}  // namespace
+
+namespace ptcore {
+
+// Vose's alias method in f64 with LIFO small/large worklists (world.rs:539-575)
+std::vector<PtSkyboxAliasEntry> alias_table(const std::vector<double>& weights) {
+    double total = 0.0;
+    for (double w : weights) total += w;
+    std::vector<PtSkyboxAliasEntry> entries;
+    if (!(total > 0.0)) return entries;
+    const size_t n = weights.size();
+    entries.resize(n);
+    std::vector<double> scaled(n);
+    std::vector<size_t> small, large;
+    for (size_t i = 0; i < n; ++i) {
+        entries[i].threshold = 1.0f;
+        entries[i].alias_index = (uint32_t)i;
+        entries[i].selection_probability = (float)(weights[i] / total);
+        scaled[i] = weights[i] / total * (double)n;
+    }
+    for (size_t i = 0; i < n; ++i) (scaled[i] < 1.0 ? small : large).push_back(i);
+    while (!small.empty() && !large.empty()) {
+        size_t low = small.back(), high = large.back();
+        small.pop_back();
+        large.pop_back();
+        entries[low].threshold = (float)std::min(std::max(scaled[low], 0.0), 1.0);
+        entries[low].alias_index = (uint32_t)high;
+        scaled[high] = (scaled[high] + scaled[low]) - 1.0;
+        (scaled[high] < 1.0 ? small : large).push_back(high);
+    }
+    return entries;
+}
+
+// GpuSceneUniforms::update_light_probabilities (world.rs:484-498)
+void update_light_probabilities(PtSceneUniforms& u, float skybox_flux) {
+    float geometry_flux = u.emissive_triangles_count > 0 ? u.total_luminous_flux : 0.0f;
+    float scale = std::fmax(geometry_flux, skybox_flux);
+    if (scale > 0.0f) {
+        float g = geometry_flux / scale, s = skybox_flux / scale;
+        u.light_probabilities[0] = g / (g + s);
+        u.light_probabilities[1] = s / (g + s);
+    } else {
+        u.light_probabilities[0] = u.light_probabilities[1] = 0.0f;
+    }
+}
+
+// Scene::uniforms_and_alias_table (world.rs:1691-1735)
+void scene_lights(const World& w, PtSceneUniforms& u, std::vector<PtSceneAliasEntry>& out) {
+    std::vector<uint32_t> indices;
+    std::vector<double> weights;
+    for (size_t i = 0; i < w.geometries.size(); ++i) {
+        const PtMaterial& m = w.materials[w.attributes[i].material];
+        double emission = luminance(m.emission_color_and_luminance) *
+                          (double)std::fmax(m.emission_color_and_luminance[3], 0.0f) *
+                          (double)std::fmax(m.emission_weight, 0.0f);
+        if (emission == 0.0) continue;
+        const float* v = w.geometries[i].v;
+        double ax = (double)v[3] - v[0], ay = (double)v[4] - v[1], az = (double)v[5] - v[2];
+        double bx = (double)v[6] - v[0], by = (double)v[7] - v[1], bz = (double)v[8] - v[2];
+        double cx = ay * bz - by * az, cy = az * bx - bz * ax, cz = ax * by - bx * ay;
+        double area = std::sqrt((cx * cx + cy * cy) + cz * cz) * 0.5;
+        double flux = 2.0 * M_PI * area * emission;
+        if (flux > 0.0 && std::isfinite(flux)) {
+            indices.push_back((uint32_t)i);
+            weights.push_back(flux);
+        }
+    }
+    double total = 0.0;
+    for (double x : weights) total += x;
+    u.total_luminous_flux = (float)std::min(total, (double)std::numeric_limits<float>::max());
+    u.emissive_triangles_count = (uint32_t)indices.size();
+    u.light_probabilities[0] = u.light_probabilities[1] = 0.0f;
+    std::vector<PtSkyboxAliasEntry> t = alias_table(weights);
+    out.resize(t.size());
+    for (size_t i = 0; i < t.size(); ++i)
+        out[i] = PtSceneAliasEntry{t[i].threshold, t[i].alias_index, indices[i], t[i].selection_probability};
+}
+
+// Camera::gpu_camera (world.rs:2159-2173) with glam's Mat3 products
+PtCamera gpu_camera(const float pos[3], float yaw, float pitch, float fov, float aspect) {
+    float sy = sinf(yaw), cy = cosf(yaw), sp = sinf(pitch), cp = cosf(pitch);
+    // columns of Ry and Rx
+    float ry[3][3] = {{cy, 0.0f, -sy}, {0.0f, 1.0f, 0.0f}, {sy, 0.0f, cy}};
+    float rx[3][3] = {{1.0f, 0.0f, 0.0f}, {0.0f, cp, sp}, {0.0f, -sp, cp}};
+    auto mulv = [](const float m[3][3], const float v[3], float out[3]) {
+        for (int k = 0; k < 3; ++k) out[k] = (m[0][k] * v[0] + m[1][k] * v[1]) + m[2][k] * v[2];
+    };
+    float rot[3][3];
+    for (int c = 0; c < 3; ++c) mulv(ry, rx[c], rot[c]);
+    const float X[3] = {1.0f, 0.0f, 0.0f}, NY[3] = {0.0f, -1.0f, 0.0f}, NZ[3] = {0.0f, 0.0f, -1.0f};
+    float height = 2.0f * tanf(fov * 0.5f);
+    float rxv[3], rny[3], rnz[3];
+    mulv(rot, X, rxv);
+    mulv(rot, NY, rny);
+    mulv(rot, NZ, rnz);
+    PtCamera c{};
+    for (int k = 0; k < 3; ++k) {
+        float h = rxv[k] * aspect * height;
+        float v = rny[k] * height;
+        c.position[k] = pos[k];
+        c.horizontal[k] = h;
+        c.vertical[k] = v;
+    }
+    for (int k = 0; k < 3; ++k)
+        c.lower_left_corner_minus_position[k] = rnz[k] - (c.horizontal[k] + c.vertical[k]) * 0.5f;
+    return c;
+}
+
+}  // namespace ptcore
+
+using namespace ptcore;
+
+extern "C" {
+
+pt_status pt_world_build(const pt_world_desc* d, pt_world** out) {
+    if (!d || !out) return set_error(PT_ERR_INVALID, "pt_world_build: null argument");
+    *out = nullptr;
+    if (d->triangle_count == 0) return set_error(PT_ERR_INVALID, "scene has no triangles");
+    if (!d->geometries || !d->attributes) return set_error(PT_ERR_INVALID, "missing triangle arrays");
+    if (d->material_count == 0 || !d->materials) return set_error(PT_ERR_INVALID, "scene has no materials");
+    for (uint32_t i = 0; i < d->triangle_count; ++i)
+        if (d->attributes[i].material >= d->material_count)
+            return set_error(PT_ERR_INVALID, "triangle %u uses material %u of %u", i, d->attributes[i].material,
+                             d->material_count);
+    if ((d->srgb_atlas_count || d->linear_atlas_count) && d->atlas_size == 0)
+        return set_error(PT_ERR_INVALID, "atlas_size is 0");
+    for (uint32_t i = 0; i < d->material_count; ++i) {
+        const PtMaterial& m = d->materials[i];
+        int32_t srgb_layers[3] = {m.base_color_tex_layer, m.emission_tex_layer, m.specular_color_tex_layer};
+        int32_t lin_layers[3] = {m.normal_tex_layer, m.metallic_roughness_tex_layer, m.specular_weight_tex_layer};
+        for (int k = 0; k < 3; ++k) {
+            if (srgb_layers[k] >= (int32_t)d->srgb_atlas_count)
+                return set_error(PT_ERR_INVALID, "material %u: sRGB atlas layer %d out of range", i, srgb_layers[k]);
+            if (lin_layers[k] >= (int32_t)d->linear_atlas_count)
+                return set_error(PT_ERR_INVALID, "material %u: linear atlas layer %d out of range", i, lin_layers[k]);
+        }
+    }
+    if (!d->skybox_rgba || d->skybox_width == 0 || d->skybox_height == 0)
+        return set_error(PT_ERR_INVALID, "missing skybox");
+    pt_world* w = new (std::nothrow) pt_world();
+    if (!w) return set_error(PT_ERR_MEMORY, "out of host memory");
+    try {
+        const uint32_t n = d->triangle_count;
+        // BVH triangles in input order (world.rs:1290-1297)
+        std::vector<BvhTri> tris(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            const float* v = d->geometries[i].v;
+            BvhTri& t = tris[i];
+            for (int k = 0; k < 3; ++k) {
+                t.box.mn[k] = std::fmin(std::fmin(v[k], v[3 + k]), v[6 + k]);
+                t.box.mx[k] = std::fmax(std::fmax(v[k], v[3 + k]), v[6 + k]);
+                t.centroid[k] = ((v[k] + v[3 + k]) + v[6 + k]) / 3.0f;
+            }
+            t.original = i;
+        }
+        w->bvh_nodes.reserve(2 * (size_t)n);
+        w->bvh_nodes.push_back(PtBvhNode{});
+        Builder b{w->bvh_nodes, tris};
+        b.set_box(0, 0, n);
+        b.subdivide_all(n);
+        w->geometries.resize(n);
+        w->attributes.resize(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            w->geometries[i] = d->geometries[tris[i].original];
+            w->attributes[i] = d->attributes[tris[i].original];
+        }
+        w->materials.assign(d->materials, d->materials + d->material_count);
+        w->atlas_size = d->atlas_size;
+        const size_t layer_bytes = (size_t)d->atlas_size * d->atlas_size * 4;
+        if (d->srgb_atlas_count) w->srgb.assign(d->srgb_atlases, d->srgb_atlases + layer_bytes * d->srgb_atlas_count);
+        if (d->linear_atlas_count)
+            w->linear.assign(d->linear_atlases, d->linear_atlases + layer_bytes * d->linear_atlas_count);
+        w->srgb_count = d->srgb_atlas_count;
+        w->linear_count = d->linear_atlas_count;
+
+        // Skybox::new (world.rs:1872-1957)
+        const uint32_t W = d->skybox_width, H = d->skybox_height;
+        std::vector<float> px(d->skybox_rgba, d->skybox_rgba + (size_t)W * H * 4);
+        double total_luma = 0.0;
+        const double total_weight = 2.0 * (double)W;
+        for (uint32_t y = 0; y < H; ++y) {
+            double wgt = row_weight(y, H);
+            for (uint32_t x = 0; x < W; ++x) total_luma = std::fma(luminance(&px[((size_t)y * W + x) * 4]), wgt, total_luma);
+        }
+        if (total_luma == 0.0) {
+            w->sky.assign(4, 0.0f);
+            w->skybox = PtSkyboxUniforms{0.0f, 0.0f, 1u, 1u};
+        } else {
+            double avg = total_luma / total_weight;
+            double scale = 1.0 / avg;
+            for (size_t i = 0; i < px.size(); i += 4)
+                for (int c = 0; c < 3; ++c) px[i + c] = (float)((double)px[i + c] * scale);
+            w->sky = std::move(px);
+            w->skybox = PtSkyboxUniforms{0.0f, (float)avg, W, H};
+        }
+        // World::edit_skybox_average_luminance (world.rs:646-648)
+        if (d->skybox_luminance >= 0.0f) w->skybox.average_luminance = std::fmax(d->skybox_luminance, 0.0f);
+
+        // World::take_updates ordering (world.rs:833-848)
+        scene_lights(*w, w->scene_uniforms, w->scene_alias);
+        if (total_luma != 0.0) {
+            std::vector<double> weights;
+            weights.reserve((size_t)W * H);
+            for (uint32_t y = 0; y < H; ++y) {
+                double wgt = row_weight(y, H);
+                for (uint32_t x = 0; x < W; ++x) weights.push_back(luminance(&w->sky[((size_t)y * W + x) * 4]) * wgt);
+            }
+            w->sky_alias = alias_table(weights);
+        }
+        const PtBvhNode& root = w->bvh_nodes[0];
+        double dx = (double)root.aabb_max[0] - root.aabb_min[0], dy = (double)root.aabb_max[1] - root.aabb_min[1],
+               dz = (double)root.aabb_max[2] - root.aabb_min[2];
+        double radius_squared = ((dx * dx + dy * dy) + dz * dz) * 0.25;
+        w->skybox.total_luminous_flux =
+            w->sky_alias.empty()
+                ? 0.0f
+                : (float)std::min(4.0 * (M_PI * M_PI) * radius_squared * (double)w->skybox.average_luminance,
+                                  (double)std::numeric_limits<float>::max());
+        update_light_probabilities(w->scene_uniforms, w->skybox.total_luminous_flux);
+
+        // camera: Camera::new from the root box, then the headless options (headless.rs:193-202)
+        const float* mn = root.aabb_min;
+        const float* mx = root.aabb_max;
+        float pos[3] = {(mn[0] + mx[0]) * 0.5f, (mn[1] + mx[1]) * 0.5f,
+                        std::fmaf(std::fmax(mx[0] - mn[0], mx[1] - mn[1]), 0.6f, mx[2])};
+        if (!std::isnan(d->camera_position[0]))
+            for (int k = 0; k < 3; ++k) pos[k] = d->camera_position[k];
+        const float rads_per_deg = 3.14159265358979323846f / 180.0f;
+        float yaw = std::isnan(d->camera_yaw_degrees) ? 0.0f : d->camera_yaw_degrees * rads_per_deg;
+        float pitch = std::isnan(d->camera_pitch_degrees) ? 0.0f : d->camera_pitch_degrees * rads_per_deg;
+        const float half_pi = 1.57079632679489661923f;
+        pitch = std::fmin(std::fmax(pitch, -half_pi), half_pi);
+        float fov = half_pi;
+        if (!std::isnan(d->camera_fov_degrees)) {
+            fov = d->camera_fov_degrees * rads_per_deg;
+            fov = std::fmin(std::fmax(fov, 0.0f), 3.14159265358979323846f - std::numeric_limits<float>::epsilon());
+        }
+        float aspect = (std::isfinite(d->aspect_ratio) && d->aspect_ratio > 0.0f) ? d->aspect_ratio : 1.0f;
+        w->camera = gpu_camera(pos, yaw, pitch, fov, aspect);
+    } catch (const std::bad_alloc&) {
+        delete w;
+        return set_error(PT_ERR_MEMORY, "out of host memory while building the world");
+    }
+    *out = w;
+    return PT_OK;
+}
+
+pt_status pt_world_destroy(pt_world* w) {
+    delete w;
+    return PT_OK;
+}
+
+pt_status pt_world_scene_desc(const pt_world* wp, pt_scene_desc* s) {
+    if (!wp || !s) return set_error(PT_ERR_INVALID, "pt_world_scene_desc: null argument");
+    const World& w = *wp;
+    std::memset(s, 0, sizeof(*s));
+    s->geometries = w.geometries.data();
+    s->attributes = w.attributes.data();
+    s->triangle_count = (uint32_t)w.geometries.size();
+    s->materials = w.materials.data();
+    s->material_count = (uint32_t)w.materials.size();
+    s->bvh_nodes = w.bvh_nodes.data();
+    s->bvh_node_count = (uint32_t)w.bvh_nodes.size();
+    s->srgb_atlases = w.srgb.empty() ? nullptr : w.srgb.data();
+    s->srgb_atlas_count = w.srgb_count;
+    s->linear_atlases = w.linear.empty() ? nullptr : w.linear.data();
+    s->linear_atlas_count = w.linear_count;
+    s->atlas_size = w.atlas_size;
+    s->camera = w.camera;
+    s->skybox_rgba = w.sky.data();
+    s->skybox = w.skybox;
+    s->skybox_alias = w.sky_alias.empty() ? nullptr : w.sky_alias.data();
+    s->skybox_alias_count = (uint32_t)w.sky_alias.size();
+    s->scene_uniforms = w.scene_uniforms;
+    s->scene_alias = w.scene_alias.empty() ? nullptr : w.scene_alias.data();
+    s->scene_alias_count = (uint32_t)w.scene_alias.size();
+    return PT_OK;
+}
+
+}  // extern "C"

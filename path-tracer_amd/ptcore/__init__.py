@@ -1,0 +1,517 @@
+"""ptcore - Python binding of libptcore.so, the MI355X path-tracing core.
+
+A thin ctypes layer over the C ABI in include/pt_api.h, mirroring the
+reference's host objects around its trace kernel:
+
+    Context  <- gpu::request_adapter/request_device   (src/render/gpu.rs:14-76)
+    World    <- World::new + take_updates              (src/render/world.rs:589-884)
+    Scene    <- WorldResources                         (src/render/world_resources.rs)
+    Tracer   <- PathTracer::{new, set_tile, set_max_bounces, trace, display,
+                sample_count}                          (src/render/path_tracer.rs:59-358)
+
+The HIP library is required: importing works without it (so CPU-only tools
+can load the structs), but every call that needs the library raises
+PtcoreUnavailable when it is missing. There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libptcore.so")
+
+
+class PtcoreUnavailable(RuntimeError):
+    """libptcore.so is not built or cannot be loaded."""
+
+
+class PtError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"ptcore error {code}: {message}")
+        self.code = code
+
+
+# status codes (pt_api.h)
+PT_OK, PT_ERR_INVALID, PT_ERR_DEVICE, PT_ERR_MEMORY, PT_ERR_STATE, PT_ERR_TIMEOUT, PT_ERR_CANCELLED = range(7)
+SCHEDULE_MEGAKERNEL, SCHEDULE_WAVEFRONT = 0, 1
+SCENE_CORNELL, SCENE_FIELD, SCENE_LARGE, SCENE_SIZED = 1, 2, 3, 4
+EXPOSURE = np.float32(0.00002604166)
+
+
+# ---------------------------------------------------------------- layouts (pt_layout.h)
+class PtCamera(C.Structure):
+    _fields_ = [("position", C.c_float * 4), ("lower_left_corner_minus_position", C.c_float * 4),
+                ("horizontal", C.c_float * 4), ("vertical", C.c_float * 4)]
+
+
+class PtSceneUniforms(C.Structure):
+    _fields_ = [("total_luminous_flux", C.c_float), ("emissive_triangles_count", C.c_uint32),
+                ("light_probabilities", C.c_float * 2)]
+
+
+class PtSkyboxUniforms(C.Structure):
+    _fields_ = [("total_luminous_flux", C.c_float), ("average_luminance", C.c_float),
+                ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class PtTraceParameters(C.Structure):
+    _fields_ = [("image_size", C.c_uint32 * 2), ("tile_origin", C.c_uint32 * 2), ("max_bounces", C.c_uint32),
+                ("sample_count", C.c_uint32), ("pad", C.c_uint32 * 2)]
+
+
+# numpy structured dtypes of the array layouts
+GEOMETRY_DTYPE = np.dtype([("v", "<f4", (9,))])
+ATTRIBUTE_DTYPE = np.dtype([("material", "<u4"), ("n", "<f4", (9,)), ("uv", "<f4", (6,)),
+                            ("t0", "<f4", (4,)), ("t1", "<f4", (4,)), ("t2", "<f4", (4,))])
+MATERIAL_DTYPE = np.dtype([
+    ("base_color", "<f4", (4,)), ("base_weight", "<f4"), ("base_metalness", "<f4"),
+    ("base_diffuse_roughness", "<f4"), ("emission_weight", "<f4"), ("emission_color_and_luminance", "<f4", (4,)),
+    ("specular_color_and_weight", "<f4", (4,)), ("specular_roughness", "<f4"),
+    ("specular_roughness_anisotropy", "<f4"), ("specular_haze", "<f4"), ("specular_haze_spread", "<f4"),
+    ("specular_ior", "<f4"), ("specular_retroreflectivity", "<f4"), ("normal_scale", "<f4"),
+    ("alpha_mode", "<u4"), ("alpha_cutoff", "<f4"), ("pad", "<f4"),
+    ("base_color_tex_layer", "<i4"), ("normal_tex_layer", "<i4"), ("metallic_roughness_tex_layer", "<i4"),
+    ("emission_tex_layer", "<i4"), ("specular_color_tex_layer", "<i4"), ("specular_weight_tex_layer", "<i4"),
+    ("base_color_uv", "<f4", (4,)), ("normal_uv", "<f4", (4,)), ("metallic_roughness_uv", "<f4", (4,)),
+    ("emission_uv", "<f4", (4,)), ("specular_color_uv", "<f4", (4,)), ("specular_weight_uv", "<f4", (4,))])
+BVH_DTYPE = np.dtype([("aabb_min", "<f4", (3,)), ("left_first", "<u4"), ("aabb_max", "<f4", (3,)),
+                      ("tri_count", "<u4")])
+SCENE_ALIAS_DTYPE = np.dtype([("threshold", "<f4"), ("alias_index", "<u4"), ("triangle_index", "<u4"),
+                              ("selection_probability", "<f4")])
+SKY_ALIAS_DTYPE = np.dtype([("threshold", "<f4"), ("alias_index", "<u4"), ("selection_probability", "<f4")])
+assert GEOMETRY_DTYPE.itemsize == 36 and ATTRIBUTE_DTYPE.itemsize == 112 and MATERIAL_DTYPE.itemsize == 224
+assert BVH_DTYPE.itemsize == 32 and SCENE_ALIAS_DTYPE.itemsize == 16 and SKY_ALIAS_DTYPE.itemsize == 12
+
+_p = C.c_void_p
+
+
+class SceneDesc(C.Structure):
+    """pt_scene_desc: device-layout arrays of one scene (bind group 1)."""
+    _fields_ = [
+        ("geometries", _p), ("attributes", _p), ("triangle_count", C.c_uint32),
+        ("materials", _p), ("material_count", C.c_uint32),
+        ("bvh_nodes", _p), ("bvh_node_count", C.c_uint32),
+        ("srgb_atlases", _p), ("srgb_atlas_count", C.c_uint32),
+        ("linear_atlases", _p), ("linear_atlas_count", C.c_uint32),
+        ("atlas_size", C.c_uint32),
+        ("camera", PtCamera),
+        ("skybox_rgba", _p), ("skybox", PtSkyboxUniforms),
+        ("skybox_alias", _p), ("skybox_alias_count", C.c_uint32),
+        ("scene_uniforms", PtSceneUniforms),
+        ("scene_alias", _p), ("scene_alias_count", C.c_uint32),
+    ]
+
+
+class WorldDesc(C.Structure):
+    """pt_world_desc: host scene-preparation input."""
+    _fields_ = [
+        ("geometries", _p), ("attributes", _p), ("triangle_count", C.c_uint32),
+        ("materials", _p), ("material_count", C.c_uint32),
+        ("srgb_atlases", _p), ("srgb_atlas_count", C.c_uint32),
+        ("linear_atlases", _p), ("linear_atlas_count", C.c_uint32), ("atlas_size", C.c_uint32),
+        ("skybox_rgba", _p), ("skybox_width", C.c_uint32), ("skybox_height", C.c_uint32),
+        ("skybox_luminance", C.c_float),
+        ("camera_position", C.c_float * 3), ("camera_yaw_degrees", C.c_float),
+        ("camera_pitch_degrees", C.c_float), ("camera_fov_degrees", C.c_float), ("aspect_ratio", C.c_float),
+    ]
+
+
+class TraceStats(C.Structure):
+    _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("pair_tests", C.c_uint64),
+                ("tri_tests", C.c_uint64), ("extend_launches", C.c_uint64), ("extend_ms", C.c_float),
+                ("shadow_ms", C.c_float), ("shade_ms", C.c_float), ("total_ms", C.c_float)]
+
+
+# every symbol include/pt_api.h and include/pt_scenes.h declare: (name, restype, argtypes)
+_u32, _i32, _f32 = C.c_uint32, C.c_int32, C.c_float
+_SIGNATURES = [
+    ("pt_last_error", C.c_char_p, []),
+    ("pt_version", C.c_char_p, []),
+    ("pt_context_create", _i32, [C.c_int, C.POINTER(_p)]),
+    ("pt_context_destroy", _i32, [_p]),
+    ("pt_context_synchronize", _i32, [_p]),
+    ("pt_world_build", _i32, [C.POINTER(WorldDesc), C.POINTER(_p)]),
+    ("pt_world_destroy", _i32, [_p]),
+    ("pt_world_scene_desc", _i32, [_p, C.POINTER(SceneDesc)]),
+    ("pt_scene_create", _i32, [_p, C.POINTER(SceneDesc), C.POINTER(_p)]),
+    ("pt_scene_destroy", _i32, [_p]),
+    ("pt_scene_footprint_bytes", _i32, [_p, C.POINTER(C.c_uint64)]),
+    ("pt_scene_update_camera", _i32, [_p, C.POINTER(PtCamera)]),
+    ("pt_scene_update_materials", _i32, [_p, _p, _p, _u32]),
+    ("pt_scene_update_lights", _i32, [_p, C.POINTER(PtSceneUniforms), _p, _u32, C.POINTER(PtSkyboxUniforms)]),
+    ("pt_scene_intersect", _i32, [_p, _p, _u32, _p, _p, _p, _p, _p, _p]),
+    ("pt_scene_occluded", _i32, [_p, _p, _p, _p, _u32, _p]),
+    ("pt_tracer_create", _i32, [_p, _p, _u32, _u32, C.POINTER(_p)]),
+    ("pt_tracer_destroy", _i32, [_p]),
+    ("pt_tracer_set_schedule", _i32, [_p, C.c_int]),
+    ("pt_tracer_set_tile", _i32, [_p, _u32, _u32, _u32, _u32, _u32, _u32]),
+    ("pt_tracer_set_max_bounces", _i32, [_p, _u32]),
+    ("pt_tracer_set_sample_count", _i32, [_p, _u32]),
+    ("pt_tracer_sample_count", _u32, [_p]),
+    ("pt_tracer_max_bounces", _u32, [_p]),
+    ("pt_tracer_trace", _i32, [_p, _u32, C.POINTER(_u32)]),
+    ("pt_tracer_trace_all", _i32, [_p, _u32]),
+    ("pt_tracer_read_accumulation", _i32, [_p, _p]),
+    ("pt_tracer_write_accumulation", _i32, [_p, _p]),
+    ("pt_tracer_read_storage", _i32, [_p, _p]),
+    ("pt_tracer_display_rgba8", _i32, [_p, _p]),
+    ("pt_tracer_primary_hits", _i32, [_p, _u32, _p, _p]),
+    ("pt_tracer_set_profiling", _i32, [_p, C.c_int]),
+    ("pt_tracer_stats", _i32, [_p, C.POINTER(TraceStats)]),
+    ("pt_generate_scene", _i32, [C.c_int, C.c_uint64, _u32, _u32, _u32, C.POINTER(_p)]),
+    ("pt_generated_world_desc", _i32, [_p, C.POINTER(WorldDesc)]),
+    ("pt_generated_destroy", _i32, [_p]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load libptcore.so (raises PtcoreUnavailable if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PtcoreUnavailable(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        try:
+            L = C.CDLL(LIB_PATH)
+        except OSError as e:
+            raise PtcoreUnavailable(f"cannot load {LIB_PATH}: {e}") from e
+        for name, res, args in _SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return [name for name, _, _ in _SIGNATURES]
+
+
+def _check(code):
+    if code != PT_OK:
+        raise PtError(code, lib().pt_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _view(ptr, dtype, count):
+    if not ptr or count == 0:
+        return np.zeros(0, dtype=dtype)
+    buf = (C.c_char * (count * np.dtype(dtype).itemsize)).from_address(ptr)
+    return np.frombuffer(buf, dtype=dtype, count=count)
+
+
+# ---------------------------------------------------------------- host scene preparation
+class Generated:
+    """A seeded procedural scene (pt_scenes.h); owns its arrays."""
+
+    def __init__(self, kind, seed, width, height, triangles=0):
+        h = C.c_void_p()
+        _check(lib().pt_generate_scene(kind, seed, triangles, width, height, C.byref(h)))
+        self.handle = h
+        self.desc = WorldDesc()
+        _check(lib().pt_generated_world_desc(h, C.byref(self.desc)))
+
+    def close(self):
+        if self.handle:
+            lib().pt_generated_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def arrays(self):
+        d = self.desc
+        return {
+            "geometries": _view(d.geometries, GEOMETRY_DTYPE, d.triangle_count),
+            "attributes": _view(d.attributes, ATTRIBUTE_DTYPE, d.triangle_count),
+            "materials": _view(d.materials, MATERIAL_DTYPE, d.material_count),
+            "skybox": _view(d.skybox_rgba, np.float32, d.skybox_width * d.skybox_height * 4).reshape(
+                d.skybox_height, d.skybox_width, 4),
+        }
+
+
+class World:
+    """Prepared host scene: pt_world_build over a WorldDesc (keeps `keep` alive)."""
+
+    def __init__(self, desc: WorldDesc, keep=None):
+        h = C.c_void_p()
+        _check(lib().pt_world_build(C.byref(desc), C.byref(h)))
+        self.handle = h
+        self._keep = keep
+        self.desc = SceneDesc()
+        _check(lib().pt_world_scene_desc(h, C.byref(self.desc)))
+
+    @classmethod
+    def generate(cls, kind, seed, width, height, triangles=0):
+        g = Generated(kind, seed, width, height, triangles)
+        w = cls(g.desc, keep=g)
+        return w
+
+    def close(self):
+        if self.handle:
+            lib().pt_world_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def arrays(self):
+        d = self.desc
+        layer = d.atlas_size * d.atlas_size * 4
+        return {
+            "geometries": _view(d.geometries, GEOMETRY_DTYPE, d.triangle_count),
+            "attributes": _view(d.attributes, ATTRIBUTE_DTYPE, d.triangle_count),
+            "materials": _view(d.materials, MATERIAL_DTYPE, d.material_count),
+            "bvh_nodes": _view(d.bvh_nodes, BVH_DTYPE, d.bvh_node_count),
+            "srgb_atlases": _view(d.srgb_atlases, np.uint8, layer * d.srgb_atlas_count),
+            "linear_atlases": _view(d.linear_atlases, np.uint8, layer * d.linear_atlas_count),
+            "skybox": _view(d.skybox_rgba, np.float32, d.skybox.width * d.skybox.height * 4),
+            "skybox_alias": _view(d.skybox_alias, SKY_ALIAS_DTYPE, d.skybox_alias_count),
+            "scene_alias": _view(d.scene_alias, SCENE_ALIAS_DTYPE, d.scene_alias_count),
+        }
+
+
+class SceneArrays:
+    """A pt_scene_desc built from numpy arrays (e.g. a hand-made or loaded scene)."""
+
+    def __init__(self, geometries, attributes, materials, bvh_nodes, camera, skybox, skybox_uniforms,
+                 skybox_alias, scene_uniforms, scene_alias, srgb=None, linear=None, atlas_size=0):
+        self.arrays = dict(geometries=np.ascontiguousarray(geometries), attributes=np.ascontiguousarray(attributes),
+                           materials=np.ascontiguousarray(materials), bvh_nodes=np.ascontiguousarray(bvh_nodes),
+                           skybox=np.ascontiguousarray(skybox, dtype=np.float32),
+                           skybox_alias=np.ascontiguousarray(skybox_alias),
+                           scene_alias=np.ascontiguousarray(scene_alias))
+        a = self.arrays
+        d = SceneDesc()
+        d.geometries, d.attributes = _ptr(a["geometries"]), _ptr(a["attributes"])
+        d.triangle_count = len(a["geometries"])
+        d.materials, d.material_count = _ptr(a["materials"]), len(a["materials"])
+        d.bvh_nodes, d.bvh_node_count = _ptr(a["bvh_nodes"]), len(a["bvh_nodes"])
+        self._srgb = None if srgb is None else np.ascontiguousarray(srgb, dtype=np.uint8)
+        self._linear = None if linear is None else np.ascontiguousarray(linear, dtype=np.uint8)
+        layer = atlas_size * atlas_size * 4
+        d.srgb_atlases = _ptr(self._srgb)
+        d.srgb_atlas_count = 0 if self._srgb is None else self._srgb.size // max(layer, 1)
+        d.linear_atlases = _ptr(self._linear)
+        d.linear_atlas_count = 0 if self._linear is None else self._linear.size // max(layer, 1)
+        d.atlas_size = atlas_size
+        d.camera = camera
+        d.skybox_rgba = _ptr(a["skybox"])
+        d.skybox = skybox_uniforms
+        d.skybox_alias, d.skybox_alias_count = _ptr(a["skybox_alias"]), len(a["skybox_alias"])
+        d.scene_uniforms = scene_uniforms
+        d.scene_alias, d.scene_alias_count = _ptr(a["scene_alias"]), len(a["scene_alias"])
+        self.desc = d
+
+
+# ---------------------------------------------------------------- device objects
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().pt_context_create(device, C.byref(h)))
+        self.handle = h
+
+    def synchronize(self):
+        _check(lib().pt_context_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            lib().pt_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """A scene resident in device memory (pt_scene)."""
+
+    def __init__(self, ctx: Context, desc: SceneDesc):
+        h = C.c_void_p()
+        _check(lib().pt_scene_create(ctx.handle, C.byref(desc), C.byref(h)))
+        self.handle = h
+        self.ctx = ctx
+
+    def footprint(self):
+        b = C.c_uint64()
+        _check(lib().pt_scene_footprint_bytes(self.handle, C.byref(b)))
+        return b.value
+
+    def update_camera(self, camera: PtCamera):
+        _check(lib().pt_scene_update_camera(self.handle, C.byref(camera)))
+
+    def update_materials(self, indices, materials):
+        idx = np.ascontiguousarray(indices, dtype=np.uint32)
+        m = np.ascontiguousarray(materials, dtype=MATERIAL_DTYPE)
+        _check(lib().pt_scene_update_materials(self.handle, _ptr(idx), _ptr(m), len(idx)))
+
+    def intersect(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        n = len(rays)
+        hit = np.empty(n, np.int32)
+        t, v, w = (np.empty(n, np.float32) for _ in range(3))
+        pairs, tris = np.empty(n, np.uint32), np.empty(n, np.uint32)
+        _check(lib().pt_scene_intersect(self.handle, _ptr(rays), n, _ptr(hit), _ptr(t), _ptr(v), _ptr(w),
+                                        _ptr(pairs), _ptr(tris)))
+        return hit, t, v, w, pairs, tris
+
+    def occluded(self, rays, tmax, skip=None):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        n = len(rays)
+        tmax = np.ascontiguousarray(tmax, dtype=np.float32)
+        skip = np.full(n, -1, np.int32) if skip is None else np.ascontiguousarray(skip, dtype=np.int32)
+        out = np.empty(n, np.uint8)
+        _check(lib().pt_scene_occluded(self.handle, _ptr(rays), _ptr(tmax), _ptr(skip), n, _ptr(out)))
+        return out
+
+    def close(self):
+        if self.handle:
+            lib().pt_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Tracer:
+    """PathTracer equivalent over a tile (path_tracer.rs:59-358)."""
+
+    MAX_SAMPLES_PER_TRACE = 16
+
+    def __init__(self, ctx: Context, scene: Scene, tile_w, tile_h, schedule=SCHEDULE_WAVEFRONT):
+        h = C.c_void_p()
+        _check(lib().pt_tracer_create(ctx.handle, scene.handle, tile_w, tile_h, C.byref(h)))
+        self.handle = h
+        self.ctx, self.scene = ctx, scene
+        self.tile_w, self.tile_h = tile_w, tile_h
+        self.set_schedule(schedule)
+
+    def set_schedule(self, schedule):
+        _check(lib().pt_tracer_set_schedule(self.handle, schedule))
+
+    def set_tile(self, image_size, origin, size):
+        _check(lib().pt_tracer_set_tile(self.handle, image_size[0], image_size[1], origin[0], origin[1],
+                                        size[0], size[1]))
+        self.tile_w, self.tile_h = size
+
+    def set_max_bounces(self, n):
+        _check(lib().pt_tracer_set_max_bounces(self.handle, n))
+
+    def set_sample_count(self, n):
+        _check(lib().pt_tracer_set_sample_count(self.handle, n))
+
+    @property
+    def sample_count(self):
+        return lib().pt_tracer_sample_count(self.handle)
+
+    @property
+    def max_bounces(self):
+        return lib().pt_tracer_max_bounces(self.handle)
+
+    def trace(self, samples):
+        n = C.c_uint32()
+        _check(lib().pt_tracer_trace(self.handle, samples, C.byref(n)))
+        return n.value
+
+    def trace_all(self, samples):
+        _check(lib().pt_tracer_trace_all(self.handle, samples))
+
+    def accumulation(self):
+        out = np.empty((self.tile_h, self.tile_w, 4), np.float32)
+        _check(lib().pt_tracer_read_accumulation(self.handle, _ptr(out)))
+        return out
+
+    def write_accumulation(self, rgba):
+        a = np.ascontiguousarray(rgba, dtype=np.float32)
+        assert a.size == self.tile_w * self.tile_h * 4
+        _check(lib().pt_tracer_write_accumulation(self.handle, _ptr(a)))
+
+    def storage(self):
+        out = np.empty((self.tile_h, self.tile_w, 4), np.float32)
+        _check(lib().pt_tracer_read_storage(self.handle, _ptr(out)))
+        return out
+
+    def display_rgba8(self):
+        out = np.empty((self.tile_h, self.tile_w, 4), np.uint8)
+        _check(lib().pt_tracer_display_rgba8(self.handle, _ptr(out)))
+        return out
+
+    def primary_hits(self, sample_count=1):
+        tri = np.empty((self.tile_h, self.tile_w), np.int32)
+        t = np.empty((self.tile_h, self.tile_w), np.float32)
+        _check(lib().pt_tracer_primary_hits(self.handle, sample_count, _ptr(tri), _ptr(t)))
+        return tri, t
+
+    def set_profiling(self, on=True):
+        _check(lib().pt_tracer_set_profiling(self.handle, 1 if on else 0))
+
+    def stats(self):
+        s = TraceStats()
+        _check(lib().pt_tracer_stats(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in TraceStats._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().pt_tracer_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gpu_available():
+    """True if a HIP device is visible (counting devices does not initialise HIP contexts in torch)."""
+    try:
+        import torch
+        return torch.cuda.is_available() and torch.cuda.device_count() > 0
+    except Exception:
+        return False
+
+
+def make_camera(position, yaw_deg=0.0, pitch_deg=0.0, fov_deg=90.0, aspect=1.0):
+    """Camera::gpu_camera (world.rs:2159-2173) evaluated in float32, for tests and tools."""
+    f = np.float32
+    rad = f(math.pi) / f(180.0)
+    yaw, pitch, fov = f(yaw_deg) * rad, f(pitch_deg) * rad, f(fov_deg) * rad
+    sy, cy, sp, cp = f(math.sin(yaw)), f(math.cos(yaw)), f(math.sin(pitch)), f(math.cos(pitch))
+    ry = np.array([[cy, 0, -sy], [0, 1, 0], [sy, 0, cy]], np.float32)  # columns
+    rx = np.array([[1, 0, 0], [0, cp, sp], [0, -sp, cp]], np.float32)
+
+    def mulv(m, v):
+        return (m[0] * v[0] + m[1] * v[1]) + m[2] * v[2]
+
+    rot = np.array([mulv(ry, rx[c]) for c in range(3)], np.float32)
+    height = f(2.0) * f(math.tan(fov * f(0.5)))
+    h = mulv(rot, np.array([1, 0, 0], np.float32)) * f(aspect) * height
+    v = mulv(rot, np.array([0, -1, 0], np.float32)) * height
+    llc = mulv(rot, np.array([0, 0, -1], np.float32)) - (h + v) * f(0.5)
+    cam = PtCamera()
+    for k in range(3):
+        cam.position[k] = position[k]
+        cam.lower_left_corner_minus_position[k] = llc[k]
+        cam.horizontal[k] = h[k]
+        cam.vertical[k] = v[k]
+    return cam
