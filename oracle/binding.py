@@ -52,7 +52,18 @@ _SIG = [
     ("oracle_rand", C.c_float, [C.POINTER(C.c_uint32)]),
     ("oracle_seed", C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
     ("oracle_display_channel", C.c_float, [C.c_float]),
+    ("oracle_math", C.c_int, [C.c_int, _p, _p, _p, C.c_uint32]),
 ]
+
+MATH_FUNCS = {"sin": 0, "cos": 1, "acos": 2, "atan2": 3, "pow": 4, "log2": 5, "exp2": 6}
+
+
+def detmath(name, x, y=None):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros_like(x) if y is None else np.ascontiguousarray(np.broadcast_to(y, x.shape), np.float32)
+    out = np.empty_like(x)
+    assert lib().oracle_math(MATH_FUNCS[name], _ptr(x), _ptr(y), _ptr(out), x.size) == 0
+    return out
 
 _lib = None
 

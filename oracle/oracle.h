@@ -82,6 +82,9 @@ float oracle_triangle_light_pdf(const pt_scene_desc* scene, uint32_t tri, const 
                                 float distance_squared, float probability);
 float oracle_rand(uint32_t* state);
 uint32_t oracle_seed(uint32_t px, uint32_t py, uint32_t sample_count);
+/* The deterministic FP32 libm shared with the kernels (pt_math.h), over
+ * arrays: fn 0 sin, 1 cos, 2 acos, 3 atan2(x, y), 4 pow(x, y), 5 log2, 6 exp2. */
+int oracle_math(int fn, const float* x, const float* y, float* out, uint32_t n);
 /* ACES + sRGB encode of one linear value into [0, 1] (display.wgsl:19-46). */
 float oracle_display_channel(float linear);
 

@@ -1129,6 +1129,24 @@ float oracle_rand(uint32_t* state) {
 }
 uint32_t oracle_seed(uint32_t px, uint32_t py, uint32_t sample_count) { return seed_of(px, py, sample_count); }
 
+// Evaluate one deterministic-libm function over arrays (fn: 0 sin, 1 cos,
+// 2 acos, 3 atan2(x, y), 4 pow(x, y), 5 log2, 6 exp2).
+int oracle_math(int fn, const float* x, const float* y, float* out, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) {
+        switch (fn) {
+            case 0: out[i] = ptm::sin_(x[i]); break;
+            case 1: out[i] = ptm::cos_(x[i]); break;
+            case 2: out[i] = ptm::acos_(x[i]); break;
+            case 3: out[i] = ptm::atan2_(x[i], y[i]); break;
+            case 4: out[i] = ptm::pow_(x[i], y[i]); break;
+            case 5: out[i] = ptm::log2_(x[i]); break;
+            case 6: out[i] = ptm::exp2_(x[i]); break;
+            default: return 1;
+        }
+    }
+    return 0;
+}
+
 // display.wgsl:19-46: ACES filmic fit, then the sRGB OETF
 float oracle_display_channel(float c) {
     float a = 2.51f, b = 0.03f, cc = 2.43f, d = 0.59f, e = 0.14f;

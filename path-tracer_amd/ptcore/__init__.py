@@ -285,6 +285,90 @@ class World:
         }
 
 
+def default_material(**fields):
+    """A GpuMaterial with the reference's defaults (world.rs:199-309), metalness 0."""
+    m = np.zeros(1, MATERIAL_DTYPE)[0]
+    m["base_color"] = 1.0
+    m["base_weight"] = 1.0
+    m["base_diffuse_roughness"] = 0.5
+    m["emission_weight"] = 1.0
+    m["emission_color_and_luminance"] = (0.0, 0.0, 0.0, 1.0)
+    m["specular_color_and_weight"] = 1.0
+    m["specular_roughness"] = 0.3
+    m["specular_haze_spread"] = 0.3
+    m["specular_ior"] = 1.5
+    m["normal_scale"] = 1.0
+    m["alpha_cutoff"] = 0.5
+    for k in ("base_color_tex_layer", "normal_tex_layer", "metallic_roughness_tex_layer", "emission_tex_layer",
+              "specular_color_tex_layer", "specular_weight_tex_layer"):
+        m[k] = -1
+    for k, v in fields.items():
+        m[k] = v
+    return m
+
+
+def triangles_from_vertices(verts, material_ids):
+    """Geometry + attribute arrays for flat triangles. verts: (N, 3, 3)."""
+    verts = np.asarray(verts, np.float32).reshape(-1, 3, 3)
+    n = len(verts)
+    geo = np.zeros(n, GEOMETRY_DTYPE)
+    geo["v"] = verts.reshape(n, 9)
+    attr = np.zeros(n, ATTRIBUTE_DTYPE)
+    nrm = np.cross(verts[:, 1] - verts[:, 0], verts[:, 2] - verts[:, 0])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    attr["material"] = np.asarray(material_ids, np.uint32)
+    attr["n"] = np.repeat(nrm, 3, axis=0).reshape(n, 9)
+    attr["uv"] = np.tile(np.array([0, 0, 1, 0, 0, 1], np.float32), (n, 1))
+    t = np.where(np.abs(nrm[:, :1]) < 0.9, np.array([[1.0, 0, 0]]), np.array([[0, 0, 1.0]]))
+    t = t - (t * nrm).sum(1, keepdims=True) * nrm
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    t4 = np.concatenate([t, np.ones((n, 1))], axis=1).astype(np.float32)
+    attr["t0"], attr["t1"], attr["t2"] = t4, t4, t4
+    return geo, attr
+
+
+class WorldInput:
+    """A pt_world_desc over numpy arrays (the input of pt_world_build)."""
+
+    def __init__(self, geometries, attributes, materials, skybox_rgba, skybox_luminance=-1.0,
+                 camera_position=None, yaw=None, pitch=None, fov=None, aspect=1.0,
+                 srgb=None, linear=None, atlas_size=0):
+        self.arrays = dict(geometries=np.ascontiguousarray(geometries, GEOMETRY_DTYPE),
+                           attributes=np.ascontiguousarray(attributes, ATTRIBUTE_DTYPE),
+                           materials=np.ascontiguousarray(materials, MATERIAL_DTYPE),
+                           skybox=np.ascontiguousarray(skybox_rgba, np.float32))
+        a = self.arrays
+        sky = a["skybox"]
+        assert sky.ndim == 3 and sky.shape[2] == 4
+        d = WorldDesc()
+        d.geometries, d.attributes = _ptr(a["geometries"]), _ptr(a["attributes"])
+        d.triangle_count = len(a["geometries"])
+        d.materials, d.material_count = _ptr(a["materials"]), len(a["materials"])
+        self._srgb = None if srgb is None else np.ascontiguousarray(srgb, np.uint8)
+        self._linear = None if linear is None else np.ascontiguousarray(linear, np.uint8)
+        layer = max(atlas_size * atlas_size * 4, 1)
+        d.srgb_atlases = _ptr(self._srgb)
+        d.srgb_atlas_count = 0 if self._srgb is None else self._srgb.size // layer
+        d.linear_atlases = _ptr(self._linear)
+        d.linear_atlas_count = 0 if self._linear is None else self._linear.size // layer
+        d.atlas_size = atlas_size
+        d.skybox_rgba = _ptr(sky)
+        d.skybox_height, d.skybox_width = sky.shape[0], sky.shape[1]
+        d.skybox_luminance = skybox_luminance
+        nan = float("nan")
+        pos = camera_position if camera_position is not None else (nan, nan, nan)
+        for k in range(3):
+            d.camera_position[k] = pos[k]
+        d.camera_yaw_degrees = nan if yaw is None else yaw
+        d.camera_pitch_degrees = nan if pitch is None else pitch
+        d.camera_fov_degrees = nan if fov is None else fov
+        d.aspect_ratio = aspect
+        self.desc = d
+
+    def build(self):
+        return World(self.desc, keep=self)
+
+
 class SceneArrays:
     """A pt_scene_desc built from numpy arrays (e.g. a hand-made or loaded scene)."""
 

@@ -1,0 +1,155 @@
+"""Host world preparation (C++ in libptcore) against the Python restatement
+of the reference's producers (oracle/world_oracle.py), bit for bit, plus the
+BVH invariants the reference relies on. CPU only."""
+import numpy as np
+import pytest
+
+import ptcore
+import world_oracle as wo
+
+
+def generated(kind, seed, w, h, tris=0):
+    g = ptcore.Generated(kind, seed, w, h, tris)
+    world = ptcore.World(g.desc, keep=g)
+    return g, world
+
+
+def oracle_world(g):
+    a = g.arrays()
+    d = g.desc
+    cam = dict(position=None if np.isnan(d.camera_position[0]) else list(d.camera_position),
+               yaw_deg=None if np.isnan(d.camera_yaw_degrees) else d.camera_yaw_degrees,
+               pitch_deg=None if np.isnan(d.camera_pitch_degrees) else d.camera_pitch_degrees,
+               fov_deg=None if np.isnan(d.camera_fov_degrees) else d.camera_fov_degrees,
+               aspect=d.aspect_ratio)
+    return wo.prepare_world(a["geometries"], a["attributes"], a["materials"], a["skybox"].reshape(-1),
+                            d.skybox_width, d.skybox_height, d.skybox_luminance, cam)
+
+
+@pytest.fixture(scope="module", params=[("cornell", ptcore.SCENE_CORNELL, 0), ("sized2k", ptcore.SCENE_SIZED, 2000)])
+def pair(request):
+    name, kind, tris = request.param
+    g, world = generated(kind, 11, 320, 240, tris)
+    return name, g, world, oracle_world(g)
+
+
+def test_bvh_bit_exact(pair):
+    _, _, world, ref = pair
+    got = world.arrays()["bvh_nodes"]
+    want = wo.nodes_to_array(ref["nodes"], ptcore.BVH_DTYPE)
+    assert len(got) == len(want)
+    assert got.tobytes() == want.tobytes()
+
+
+def test_reorder_bit_exact(pair):
+    _, _, world, ref = pair
+    a = world.arrays()
+    assert a["geometries"].tobytes() == np.ascontiguousarray(ref["geometries"]).tobytes()
+    assert a["attributes"].tobytes() == np.ascontiguousarray(ref["attributes"]).tobytes()
+
+
+def test_scene_lights_bit_exact(pair):
+    _, _, world, ref = pair
+    u = world.desc.scene_uniforms
+    assert np.float32(u.total_luminous_flux) == ref["total_flux"]
+    assert u.emissive_triangles_count == ref["emissive_count"]
+    got = world.arrays()["scene_alias"]
+    assert len(got) == len(ref["scene_alias"])
+    for e, (t, a, tri, s) in zip(got, ref["scene_alias"]):
+        assert (e["threshold"], e["alias_index"], e["triangle_index"], e["selection_probability"]) == (t, a, tri, s)
+    assert [np.float32(x) for x in u.light_probabilities] == ref["light_probabilities"]
+
+
+def test_sky_bit_exact(pair):
+    _, _, world, ref = pair
+    a = world.arrays()
+    assert a["skybox"].tobytes() == np.asarray(ref["sky"], np.float32).tobytes()
+    sk = world.desc.skybox
+    ru = ref["sky_uniforms"]
+    assert (sk.width, sk.height) == (ru["width"], ru["height"])
+    assert np.float32(sk.average_luminance) == ru["average_luminance"]
+    assert np.float32(sk.total_luminous_flux) == ru["total_luminous_flux"]
+    got = a["skybox_alias"]
+    assert len(got) == len(ref["sky_alias"])
+    want = np.array([tuple(e) for e in ref["sky_alias"]], dtype=ptcore.SKY_ALIAS_DTYPE)
+    assert got.tobytes() == want.tobytes()
+
+
+def test_camera_bit_exact(pair):
+    _, _, world, ref = pair
+    c = world.desc.camera
+    rc = ref["camera"]
+    for k in range(3):
+        assert np.float32(c.position[k]) == rc["position"][k]
+        assert np.float32(c.lower_left_corner_minus_position[k]) == rc["llc"][k]
+        assert np.float32(c.horizontal[k]) == rc["horizontal"][k]
+        assert np.float32(c.vertical[k]) == rc["vertical"][k]
+
+
+def check_bvh_invariants(nodes, geometries):
+    n = len(geometries)
+    seen = np.zeros(n, np.int32)
+    v = geometries["v"].reshape(-1, 3, 3)
+    stack = [0]
+    while stack:
+        i = stack.pop()
+        node = nodes[i]
+        if node["tri_count"] > 0:
+            s, c = int(node["left_first"]), int(node["tri_count"])
+            seen[s:s + c] += 1
+            tv = v[s:s + c].reshape(-1, 3)
+            assert (tv >= node["aabb_min"]).all() and (tv <= node["aabb_max"]).all()
+        else:
+            lf = int(node["left_first"])
+            for ch in (lf, lf + 1):
+                assert (nodes[ch]["aabb_min"] >= node["aabb_min"]).all()
+                assert (nodes[ch]["aabb_max"] <= node["aabb_max"]).all()
+                stack.append(ch)
+    assert (seen == 1).all(), "every triangle must be in exactly one leaf"
+
+
+@pytest.mark.parametrize("kind,tris", [(ptcore.SCENE_CORNELL, 0), (ptcore.SCENE_FIELD, 0), (ptcore.SCENE_SIZED, 50000)])
+def test_bvh_invariants(kind, tris):
+    _, world = generated(kind, 3, 640, 480, tris)
+    a = world.arrays()
+    check_bvh_invariants(a["bvh_nodes"], a["geometries"])
+    assert len(a["bvh_nodes"]) <= 2 * len(a["geometries"]) - 1
+
+
+def test_alias_table_exactness():
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 7, 100, 1000):
+        w = list(rng.lognormal(0, 2, n))
+        table = wo.alias_table(w)
+        total = sum(w)
+        mass = np.zeros(n)
+        for i, (t, a, s) in enumerate(table):
+            mass[i] += float(t)
+            mass[a] += 1.0 - float(t)
+        np.testing.assert_allclose(mass / n, np.array(w) / total, rtol=1e-6, atol=1e-7)
+
+
+def test_product_alias_tables_reproduce_weights(field):
+    """Acceptance mass of every slot reproduces selection_probability (to f32 rounding)."""
+    a = field.arrays()
+    for table in (a["scene_alias"], a["skybox_alias"]):
+        n = len(table)
+        mass = np.zeros(n)
+        np.add.at(mass, np.arange(n), table["threshold"].astype(np.float64))
+        np.add.at(mass, table["alias_index"].astype(np.int64), 1.0 - table["threshold"].astype(np.float64))
+        np.testing.assert_allclose(mass / n, table["selection_probability"], rtol=2e-5, atol=1e-7)
+
+
+def test_world_rejects_bad_input():
+    g = ptcore.Generated(ptcore.SCENE_CORNELL, 1, 64, 64)
+    d = g.desc
+    saved = d.triangle_count
+    d.triangle_count = 0
+    with pytest.raises(ptcore.PtError) as e:
+        ptcore.World(d)
+    assert e.value.code == ptcore.PT_ERR_INVALID and "no triangles" in str(e.value)
+    d.triangle_count = saved
+    a = g.arrays()
+    a["attributes"][3]["material"] = 999
+    with pytest.raises(ptcore.PtError):
+        ptcore.World(d)
