@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Benchmark of the path-tracing core on one MI355X.
+
+Workload (default): the ~262k-triangle procedural field scene (512x512-class
+noise heightfield sized to 261,920 triangles with 64 icospheres, 256 emissive
+triangles), 1920x1080, 8 samples per pixel, max bounces 6. One step = one
+render job = every sample of the frame, traced from sample 1 (the scene,
+its BVH and light tables are resident in HBM before timing starts).
+
+Prints one JSON line: path samples/s (value), rays/s, p50/p99 job latency,
+the roofline of the traversal (extend) kernel measured live with HIP events
+on its stream, and the CPU oracle's throughput on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload field|cornell|large]
+                    [--schedule wavefront|mega] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "path-tracer_amd"))
+
+HBM_PEAK_GBS = 8000.0
+PAIR_BYTES, TRI_BYTES = 64, 36  # a child pair of 32-B nodes; one 36-B triangle
+
+WORKLOADS = {
+    # name: (scene kind, width, height, spp, max_bounces, description)
+    "field": ("SCENE_FIELD", 1920, 1080, 8, 6,
+              "field-262k: 261,920-triangle noise heightfield + 64 icospheres, 256 emissive tris, 1920x1080, 8 spp, depth 6"),
+    "cornell": ("SCENE_CORNELL", 256, 256, 16, 8, "cornell-36: Cornell box, 256x256, 16 spp, depth 8"),
+    "large": ("SCENE_LARGE", 3840, 2160, 64, 8,
+              "large-4.2M: 4,195,522-triangle heightfield + 4096 icospheres, 8 Perlin textures, 3840x2160, 64 spp, depth 8"),
+}
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = (len(xs) - 1) * q / 100.0
+    lo, hi = math.floor(k), math.ceil(k)
+    return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def cpu_baseline(world, w, h, bounces, samples):
+    """The CPU oracle (C++ restatement of the reference method, OpenMP over
+    pixels) on the host cores: full frame at `samples` spp."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import binding as oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    _, st = oracle.render_rows_f32(world.desc, w, h, bounces, samples, 0, h, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": w * h * samples / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
+            "sample": f"full {w}x{h} frame at {samples} spp (of the benchmark's spp), depth {bounces}, "
+                      f"{dt:.1f} s wall; rays/s {((st['closest_rays'] + st['shadow_rays']) / dt):.4g}"}
+
+
+def load_traffic(name):
+    """HBM bytes per extend launch from the committed rocprofv3 --pmc summary, if present."""
+    p = os.path.join(ROOT, "profiles", f"extend_traffic_{name}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="field", choices=sorted(WORKLOADS))
+    ap.add_argument("--schedule", default="wavefront", choices=["wavefront", "mega"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-spp", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(pixel/tile sharding across GPUs is designed but not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import ptcore
+
+    kind_name, W, H, SPP, BOUNCES, desc = WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    world = ptcore.World.generate(getattr(ptcore, kind_name), {"field": 2, "cornell": 1, "large": 3}[args.workload], W, H)
+    prep_s = time.perf_counter() - t0
+    ctx = ptcore.Context(0)
+    t0 = time.perf_counter()
+    scene = ptcore.Scene(ctx, world.desc)
+    ctx.synchronize()
+    upload_s = time.perf_counter() - t0
+    schedule = ptcore.SCHEDULE_WAVEFRONT if args.schedule == "wavefront" else ptcore.SCHEDULE_MEGAKERNEL
+    tracer = ptcore.Tracer(ctx, scene, W, H, schedule)
+    tracer.set_max_bounces(BOUNCES)
+    print(f"[bench] {desc}; {world.desc.triangle_count} tris, host prep {prep_s:.2f}s, upload {upload_s:.2f}s",
+          file=sys.stderr, flush=True)
+
+    def step():
+        tracer.set_sample_count(0)
+        tracer.trace_all(SPP)
+
+    # counting pass: traversal work per step (rendering is deterministic, so
+    # every step issues exactly these rays and tests)
+    tracer.set_profiling(2)
+    step()
+    counts = tracer.stats()
+    print(f"[bench] counting pass: {counts}", file=sys.stderr, flush=True)
+
+    tracer.set_profiling(0)
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+
+    tracer.set_profiling(1)  # live HIP events around every kernel, no extra syncs
+    lat = []
+    ctx.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        step()
+        ctx.synchronize()
+        lat.append((time.perf_counter() - s0) * 1e3)
+    total_s = time.perf_counter() - t_start
+    ev = tracer.stats()
+    tracer.set_profiling(0)
+
+    samples_per_step = W * H * SPP
+    rays_per_step = counts["closest_rays"] + counts["shadow_rays"]
+    value = samples_per_step * args.steps / total_s
+    ms_per_step = total_s * 1e3 / args.steps
+    result = {
+        "metric": "path samples/s @1080p depth 6; rays/s; p50/p99 render-job latency",
+        "value": value,
+        "unit": "path samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": desc, "width": W, "height": H, "spp": SPP, "max_bounces": BOUNCES,
+                   "triangles": int(world.desc.triangle_count), "schedule": args.schedule},
+        "rays_per_s": rays_per_step * args.steps / total_s,
+        "p50_ms": percentile(lat, 50),
+        "p99_ms": percentile(lat, 99),
+    }
+    if schedule == ptcore.SCHEDULE_WAVEFRONT:
+        extend_launches = max(ev["extend_launches"], 1)
+        bytes_per_step = PAIR_BYTES * counts["pair_tests"] + TRI_BYTES * counts["tri_tests"]
+        launches_per_step = extend_launches / args.steps
+        bytes_per_launch = bytes_per_step / launches_per_step
+        avg_launch_s = ev["extend_ms"] / 1e3 / extend_launches
+        achieved = bytes_per_launch / avg_launch_s / 1e9
+        result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(args.workload),
+                              "kernel": "k_extend (closest-hit traversal)",
+                              "avg_launch_ms": avg_launch_s * 1e3, "launches_per_step": launches_per_step,
+                              "algorithmic_bytes_per_launch": bytes_per_launch}
+        result["kernel_ms_per_step"] = {"extend": ev["extend_ms"] / args.steps, "shade": ev["shade_ms"] / args.steps,
+                                        "shadow": ev["shadow_ms"] / args.steps, "total": ev["total_ms"] / args.steps}
+    else:
+        result["kernel_ms_per_step"] = {"megakernel": ev["total_ms"] / args.steps}
+        bytes_per_step = PAIR_BYTES * counts["pair_tests"] + TRI_BYTES * counts["tri_tests"]
+        result["roofline"] = {"bound": "hbm", "achieved": bytes_per_step / (ev["total_ms"] / 1e3 / args.steps) / 1e9,
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
+                              "kernel": "k_megakernel (all traversal bytes over the whole kernel)"}
+        result["roofline"]["frac"] = result["roofline"]["achieved"] / HBM_PEAK_GBS
+    result["traversal_counts_per_step"] = {k: counts[k] for k in ("closest_rays", "shadow_rays", "pair_tests",
+                                                                   "tri_tests", "shadow_pair_tests", "shadow_tri_tests")}
+    if not args.no_cpu_baseline:
+        print("[bench] CPU baseline (oracle on host cores)...", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(world, W, H, BOUNCES, min(args.cpu_spp, SPP))
+    else:
+        result["cpu_baseline"] = None
+    tracer.close()
+    scene.close()
+    ctx.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

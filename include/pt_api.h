@@ -197,23 +197,118 @@ pt_status pt_tracer_display_rgba8(pt_tracer* tracer, uint8_t* rgba);
  * `sample_count` (same jitter as the trace): triangle index (-1 on miss), t. */
 pt_status pt_tracer_primary_hits(pt_tracer* tracer, uint32_t sample_count, int32_t* tri, float* t);
 
-/* Device counters of the last trace call (wavefront schedule): closest-hit
- * rays, shadow rays, child-pair box tests, triangle tests, path segments. */
+/* Profiling of the trace calls made since pt_tracer_set_profiling: HIP-event
+ * time per kernel class (recorded on the context's stream around each
+ * launch, read without per-launch synchronisation) and, at level 2, device
+ * traversal counters: closest-hit and shadow rays issued, child-pair box
+ * tests (64 contiguous bytes each) and triangle tests (36 bytes each). The
+ * megakernel reports its one kernel in total_ms and all traversals in
+ * pair_tests/tri_tests. */
 typedef struct pt_trace_stats {
     uint64_t closest_rays;
     uint64_t shadow_rays;
-    uint64_t pair_tests;
+    uint64_t pair_tests;         /* closest-hit traversal */
     uint64_t tri_tests;
+    uint64_t shadow_pair_tests;  /* any-hit traversal (wavefront) */
+    uint64_t shadow_tri_tests;
     uint64_t extend_launches;
-    float extend_ms;      /* summed HIP-event time of the extend kernels */
-    float shadow_ms;      /* summed HIP-event time of the shadow kernels */
+    float extend_ms;             /* summed HIP-event time of the extend kernels */
+    float shadow_ms;             /* summed HIP-event time of the shadow kernels */
     float shade_ms;
-    float total_ms;
+    float total_ms;              /* whole trace calls */
 } pt_trace_stats;
-/* Enable/disable device counters and per-kernel event timing (off by default;
- * on costs a few percent). */
-pt_status pt_tracer_set_profiling(pt_tracer* tracer, int enabled);
+/* level 0 off, 1 per-kernel event timing, 2 timing + device counters
+ * (counting kernels; synchronises once per chunk). Resets the statistics. */
+pt_status pt_tracer_set_profiling(pt_tracer* tracer, int level);
+/* Synchronises and returns the statistics accumulated since set_profiling. */
 pt_status pt_tracer_stats(pt_tracer* tracer, pt_trace_stats* out);
+
+/* ---------------------------------------------------------------- batching */
+/* Wavefront tuning of a context, used by every wavefront render on it. */
+typedef struct pt_wave_config {
+    uint32_t lds_stack_depth;  /* traversal stack entries kept in LDS: 16, 24 or 32 */
+    uint32_t chunk_paths;      /* paths in flight per chunk (the queue tile size) */
+    uint32_t sort_key;         /* ray-queue order: 0 none, 1 direction octant, 2 hit material */
+} pt_wave_config;
+#define PT_SORT_NONE 0
+#define PT_SORT_OCTANT 1
+#define PT_SORT_MATERIAL 2
+pt_status pt_context_set_wave_config(pt_context* ctx, const pt_wave_config* config);
+pt_status pt_context_get_wave_config(const pt_context* ctx, pt_wave_config* config);
+
+/* Trace samples[i] more samples on each tracers[i] in one wavefront launch
+ * sequence whose queues mix every job: scenes, tile sizes and sample counts
+ * may all differ (all tracers on one context). Per tracer the result equals
+ * pt_tracer_trace_all(tracers[i], samples[i]) bit for bit. Asynchronous. */
+pt_status pt_trace_batch(pt_tracer* const* tracers, uint32_t count, const uint32_t* samples);
+
+/* Autotuner over the batch: times it under wave configurations (coordinate
+ * descent over LDS stack depth {16, 24, 32}, chunk size {1M, 2M, 4M, 8M}
+ * paths and sort key {none, octant, material}), keeps the fastest in the
+ * context and reports it. Every trial restarts the tracers at sample 1; they
+ * are left reset (sample_count 0). */
+typedef struct pt_tune_result {
+    pt_wave_config best;
+    float best_ms;      /* batch time under the best configuration */
+    float initial_ms;   /* batch time under the configuration it started from */
+    uint32_t trials;
+} pt_tune_result;
+pt_status pt_autotune_batch(pt_tracer* const* tracers, uint32_t count, const uint32_t* samples, pt_tune_result* out);
+
+/* ------------------------------------------------------------------ serving */
+/* A render server on a context: a worker thread that keeps scenes resident,
+ * admits queued jobs while slots and the HBM budget allow, and traces them
+ * with continuous batching: each round is one wavefront chunk made of pixel
+ * blocks from every active job, and jobs submitted meanwhile join the next
+ * round. A job's pixels always take all of their samples within one round,
+ * in sample order, so its image is bit-identical however it is mixed. */
+typedef struct pt_server pt_server;
+typedef struct pt_server_config {
+    uint64_t hbm_budget_bytes;  /* admission budget: resident scenes + job buffers + path state */
+    uint32_t max_active_jobs;   /* job slots */
+    uint32_t round_paths;       /* paths per round (0: the context's chunk size) */
+} pt_server_config;
+typedef struct pt_render_request {
+    pt_scene* scene;            /* a scene of the server's context */
+    uint32_t width, height;     /* the whole image is one tile */
+    uint32_t samples;           /* samples per pixel, from sample 1 */
+    uint32_t max_bounces;       /* 0..16 */
+} pt_render_request;
+#define PT_JOB_QUEUED 0
+#define PT_JOB_RUNNING 1
+#define PT_JOB_DONE 2
+#define PT_JOB_FAILED 3
+#define PT_JOB_CANCELLED 4
+typedef struct pt_job_info {
+    int32_t state;
+    uint32_t rounds;            /* rounds the job took part in */
+    double submit_s;            /* steady-clock seconds */
+    double admit_s;
+    double done_s;              /* result ready (0 until done) */
+    uint64_t footprint_bytes;   /* device bytes held by the job */
+} pt_job_info;
+typedef struct pt_server_stats {
+    uint64_t rounds;
+    uint64_t paths;             /* path samples traced */
+    uint64_t jobs_done;
+    uint64_t jobs_rejected;
+    double busy_s;              /* time the worker spent in rounds */
+    uint64_t bytes_in_use;      /* admitted footprint now */
+} pt_server_stats;
+pt_status pt_server_create(pt_context* ctx, const pt_server_config* config, pt_server** out);
+/* Cancels queued jobs, finishes the running round, joins the worker. */
+pt_status pt_server_destroy(pt_server* server);
+/* Queue a job; PT_ERR_MEMORY if it can never fit the budget. Thread-safe. */
+pt_status pt_server_submit(pt_server* server, const pt_render_request* request, uint64_t* job_id);
+/* Wait until the job is done, failed or cancelled; PT_ERR_TIMEOUT after timeout_ms. */
+pt_status pt_server_wait(pt_server* server, uint64_t job_id, uint32_t timeout_ms);
+pt_status pt_server_info(pt_server* server, uint64_t job_id, pt_job_info* info);
+/* Copy a done job's accumulation (width*height float4, the sum of its
+ * samples) to host memory and release the job. */
+pt_status pt_server_result(pt_server* server, uint64_t job_id, float* rgba);
+/* Cancel a queued job (running jobs finish their current round first). */
+pt_status pt_server_cancel(pt_server* server, uint64_t job_id);
+pt_status pt_server_stats_get(pt_server* server, pt_server_stats* out);
 
 #ifdef __cplusplus
 }  /* extern "C" */

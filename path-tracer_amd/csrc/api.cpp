@@ -20,6 +20,7 @@
 #include "kernels.h"
 #include "pt_internal.h"
 #include "pt_runtime.h"
+#include "wavefront.h"
 
 namespace ptcore {
 
@@ -215,6 +216,7 @@ pt_status pt_context_destroy(pt_context* c) {
     hipFree(c->ovf);
     hipFree(c->srgb_lut);
     hipFree(c->counters);
+    delete c->batch_wave;
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return PT_OK;

@@ -15,6 +15,10 @@
 // order (emission, then NEE, then the next bounce), so each path's result
 // is bit-identical to the megakernel's. Scenes with alpha-tested materials
 // run the shadow test inline in shade to keep the draw order.
+//
+// Work comes from a job table: one job for a tracer tile, or many jobs of
+// different scenes, resolutions and sample counts for a batch. Paths of
+// different jobs share the queues; MULTI kernels look up each path's scene.
 #include "wavefront.h"
 
 #include <algorithm>
@@ -24,6 +28,7 @@ namespace ptd {
 
 constexpr uint32_t WAVE_BLOCK = 256;
 constexpr uint32_t MAX_COUNTS = 2 * (PT_MAX_BOUNCES_LIMIT + 2);
+constexpr uint32_t SORT_BUCKETS = 256;
 
 __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(0xffffffffu, __builtin_amdgcn_mbcnt_lo(0xffffffffu, 0u));
@@ -59,34 +64,66 @@ __device__ __forceinline__ void st3(float* x, float* y, float* z, uint32_t i, f3
     z[i] = v.z;
 }
 
+// scene and job of path p: the kernel arguments for a single job, the tables otherwise
+template <bool MULTI>
+__device__ __forceinline__ const DevScene& scene_of(const DevScene& s0, const DevScene* scenes, const WaveJob* jobs,
+                                                    uint32_t job) {
+    if (MULTI) return scenes[jobs[job].scene];
+    return s0;
+}
+template <bool MULTI>
+__device__ __forceinline__ const WaveJob& job_of(const WaveJob& j0, const WaveJob* jobs, uint32_t job) {
+    if (MULTI) return jobs[job];
+    return j0;
+}
+
 __global__ void k_zero_counts(uint32_t* counts) {
     if (threadIdx.x < MAX_COUNTS) counts[threadIdx.x] = 0u;
 }
 
-__global__ __launch_bounds__(WAVE_BLOCK) void k_generate(DevScene s, WaveArgs a, PathState ps, uint32_t* __restrict__ queue,
-                                                         uint32_t* __restrict__ counts) {
-    const uint32_t n = a.chunk_pixels * a.tile.samples;
+// unit of path i: the last unit whose offset is <= i
+__device__ __forceinline__ uint32_t find_unit(const ChunkUnits& u, uint32_t i) {
+    uint32_t lo = 0, hi = u.n_units;  // offset[lo] <= i < offset[hi]
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (u.offset[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(WAVE_BLOCK) void k_generate(DevScene s0, WaveJob j0, const DevScene* __restrict__ scenes,
+                                                         const WaveJob* __restrict__ jobs, ChunkUnits units, PathState ps,
+                                                         uint32_t* __restrict__ queue, uint32_t* __restrict__ counts) {
+    const uint32_t n = units.n_paths;
     const uint32_t stride = gridDim.x * WAVE_BLOCK;
     for (uint32_t base = blockIdx.x * WAVE_BLOCK; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
         bool active = false;
         if (i < n) {
-            const uint32_t k = i / a.chunk_pixels, q = i % a.chunk_pixels;
+            const uint32_t u = find_unit(units, i);
+            const uint32_t ji = units.job[u];
+            const WaveJob& jb = job_of<MULTI>(j0, jobs, ji);
+            const uint32_t rel = i - units.offset[u];
+            const uint32_t k = rel >> 6, lane = rel & 63u;
             uint32_t gx, gy;
-            block_pixel(a.chunk_first_pixel + q, a.tile.tile_w, gx, gy);
-            const uint32_t px = gx + a.tile.origin_x, py = gy + a.tile.origin_y;
-            active = gx < a.tile.tile_w && gy < a.tile.tile_h && px < a.tile.image_w && py < a.tile.image_h;
+            block_pixel(units.block[u] * 64u + lane, jb.tile_w, gx, gy);
+            const uint32_t px = gx + jb.origin_x, py = gy + jb.origin_y;
+            active = gx < jb.tile_w && gy < jb.tile_h && px < jb.image_w && py < jb.image_h;
             ps.lr[i] = 0.0f;
             ps.lg[i] = 0.0f;
             ps.lb[i] = 0.0f;
             if (active) {
-                Rng rng{ptm::rng_seed(px, py, a.tile.first_sample + k)};
-                f3 d = primary_direction(s, px, py, a.tile.image_w, a.tile.image_h, rng);
+                const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, ji);
+                Rng rng{ptm::rng_seed(px, py, jb.first_sample + k)};
+                f3 d = primary_direction(s, px, py, jb.image_w, jb.image_h, rng);
                 st3(ps.ox, ps.oy, ps.oz, i, f3{s.camera.position[0], s.camera.position[1], s.camera.position[2]});
                 st3(ps.dx, ps.dy, ps.dz, i, d);
                 st3(ps.tr, ps.tg, ps.tb, i, f3{1.0f, 1.0f, 1.0f});
                 ps.prev_pdf[i] = 0.0f;
                 ps.rng[i] = rng.s;
+                ps.job[i] = ji;
             }
         }
         const uint32_t slot = wave_append(&counts[0], active);
@@ -94,8 +131,10 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_generate(DevScene s, WaveArgs a,
     }
 }
 
-template <uint32_t LDS_DEPTH, bool COUNT>
-__global__ __launch_bounds__(WAVE_BLOCK) void k_extend(DevScene s, PathState ps, const uint32_t* __restrict__ queue,
+template <uint32_t LDS_DEPTH, bool COUNT, bool MULTI>
+__global__ __launch_bounds__(WAVE_BLOCK) void k_extend(DevScene s0, const DevScene* __restrict__ scenes,
+                                                       const WaveJob* __restrict__ jobs, PathState ps,
+                                                       const uint32_t* __restrict__ queue,
                                                        const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ ovf,
                                                        unsigned long long* __restrict__ counters) {
     __shared__ uint32_t lds_stack[LDS_DEPTH * WAVE_BLOCK];
@@ -110,16 +149,18 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_extend(DevScene s, PathState ps,
     TraversalCounters ctr = {0, 0};
     for (uint32_t j = blockIdx.x * WAVE_BLOCK + threadIdx.x; j < count; j += gridDim.x * WAVE_BLOCK) {
         const uint32_t p = queue[j];
+        const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
         const f3 o = ld3(ps.ox, ps.oy, ps.oz, p), d = ld3(ps.dx, ps.dy, ps.dz, p);
         const f3 rcp = f3{1.0f, 1.0f, 1.0f} / d;
-        Rng rng{s.all_opaque ? 0u : ps.rng[p]};
+        const bool opaque = s.all_opaque != 0u;
+        Rng rng{opaque ? 0u : ps.rng[p]};
         float t, v, w;
         const int32_t h = closest_hit<COUNT>(s, o, d, rcp, rng, st, t, v, w, ctr);
         ps.hit[p] = h;
         ps.ht[p] = t;
         ps.hv[p] = v;
         ps.hw[p] = w;
-        if (!s.all_opaque) ps.rng[p] = rng.s;
+        if (!opaque) ps.rng[p] = rng.s;
     }
     if (COUNT) {
         atomicAdd(&counters[2], (unsigned long long)ctr.pairs);
@@ -127,8 +168,10 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_extend(DevScene s, PathState ps,
     }
 }
 
-template <uint32_t LDS_DEPTH, bool COUNT>
-__global__ __launch_bounds__(WAVE_BLOCK) void k_shadow(DevScene s, PathState ps, const uint32_t* __restrict__ queue,
+template <uint32_t LDS_DEPTH, bool COUNT, bool MULTI>
+__global__ __launch_bounds__(WAVE_BLOCK) void k_shadow(DevScene s0, const DevScene* __restrict__ scenes,
+                                                       const WaveJob* __restrict__ jobs, PathState ps,
+                                                       const uint32_t* __restrict__ queue,
                                                        const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ ovf,
                                                        unsigned long long* __restrict__ counters) {
     __shared__ uint32_t lds_stack[LDS_DEPTH * WAVE_BLOCK];
@@ -143,36 +186,40 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_shadow(DevScene s, PathState ps,
     TraversalCounters ctr = {0, 0};
     for (uint32_t j = blockIdx.x * WAVE_BLOCK + threadIdx.x; j < count; j += gridDim.x * WAVE_BLOCK) {
         const uint32_t p = queue[j];
+        const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
         const f3 o = ld3(ps.sox, ps.soy, ps.soz, p), d = ld3(ps.sdx, ps.sdy, ps.sdz, p);
-        Rng rng{0u};  // opaque scenes only: any-hit draws nothing
+        Rng rng{0u};  // only opaque scenes queue shadow rays: any-hit draws nothing
         if (!occluded<COUNT>(s, o, d, ps.sdist[p], ps.slight[p], rng, st, ctr)) {
             f3 L = ld3(ps.lr, ps.lg, ps.lb, p) + ld3(ps.scr, ps.scg, ps.scb, p);
             st3(ps.lr, ps.lg, ps.lb, p, L);
         }
     }
     if (COUNT) {
-        atomicAdd(&counters[2], (unsigned long long)ctr.pairs);
-        atomicAdd(&counters[3], (unsigned long long)ctr.tris);
+        atomicAdd(&counters[4], (unsigned long long)ctr.pairs);
+        atomicAdd(&counters[5], (unsigned long long)ctr.tris);
     }
 }
 
 // One bounce of shading for the paths in `queue` (trace.slang:886-1296).
-template <bool INLINE_SHADOW>
-__global__ __launch_bounds__(WAVE_BLOCK) void k_shade(DevScene s, PathState ps, uint32_t bounce, uint32_t max_bounces,
-                                                      const uint32_t* __restrict__ queue, const uint32_t* __restrict__ count_ptr,
+// Paths of scenes with alpha-tested materials test their shadow ray inline
+// (the any-hit draws random numbers between NEE and BSDF sampling).
+template <bool MULTI>
+__global__ __launch_bounds__(WAVE_BLOCK) void k_shade(DevScene s0, const DevScene* __restrict__ scenes,
+                                                      const WaveJob* __restrict__ jobs, WaveJob j0, PathState ps,
+                                                      uint32_t bounce, const uint32_t* __restrict__ queue,
+                                                      const uint32_t* __restrict__ count_ptr,
                                                       uint32_t* __restrict__ next_queue, uint32_t* __restrict__ next_count,
                                                       uint32_t* __restrict__ shadow_queue, uint32_t* __restrict__ shadow_count,
                                                       uint32_t* __restrict__ ovf) {
-    __shared__ uint32_t lds_stack[INLINE_SHADOW ? MEGA_LDS_DEPTH * WAVE_BLOCK : 1];
+    __shared__ uint32_t lds_stack[MEGA_LDS_DEPTH * WAVE_BLOCK];
     Stack st;
-    st.lds = lds_stack + (INLINE_SHADOW ? threadIdx.x : 0u);
+    st.lds = lds_stack + threadIdx.x;
     st.lds_stride = WAVE_BLOCK;
     st.lds_depth = MEGA_LDS_DEPTH;
     st.ovf = ovf;
     st.ovf_stride = gridDim.x * WAVE_BLOCK;
     st.gtid = blockIdx.x * WAVE_BLOCK + threadIdx.x;
     const uint32_t count = *count_ptr;
-    const float prob_sky = s.uniforms.light_probabilities[1];
     const uint32_t stride = gridDim.x * WAVE_BLOCK;
     for (uint32_t base = blockIdx.x * WAVE_BLOCK; base < count; base += stride) {
         const uint32_t j = base + threadIdx.x;
@@ -180,15 +227,19 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_shade(DevScene s, PathState ps, 
         uint32_t p = 0;
         if (j < count) {
             p = queue[j];
+            const uint32_t ji = MULTI ? ps.job[p] : 0u;
+            const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, ji);
+            const uint32_t max_bounces = job_of<MULTI>(j0, jobs, ji).max_bounces;
             const f3 ray_origin = ld3(ps.ox, ps.oy, ps.oz, p), ray_direction = ld3(ps.dx, ps.dy, ps.dz, p);
             f3 throughput = ld3(ps.tr, ps.tg, ps.tb, p);
             f3 L = ld3(ps.lr, ps.lg, ps.lb, p);
-            float previous_bsdf_pdf = ps.prev_pdf[p];
+            const float previous_bsdf_pdf = ps.prev_pdf[p];
             Rng rng{ps.rng[p]};
             const int32_t hit = ps.hit[p];
             if (hit == -1) {
                 float mis = 1.0f;
-                if (bounce > 0u) mis = power_heuristic(previous_bsdf_pdf, skybox_light_pdf(s, ray_direction, prob_sky));
+                if (bounce > 0u)
+                    mis = power_heuristic(previous_bsdf_pdf, skybox_light_pdf(s, ray_direction, s.uniforms.light_probabilities[1]));
                 L = L + throughput * evaluate_skybox(s, ray_direction) * mis;
             } else {
                 SurfaceHit sf;
@@ -206,7 +257,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_shade(DevScene s, PathState ps, 
                     evaluate_lobes(b, sf.wo, lwi, f, pdf);
                     const float weight = bounce < max_bounces ? power_heuristic(light.pdf, pdf) : 1.0f;
                     const f3 contrib = throughput * f * lwi.z * light.emission * (weight / light.pdf);
-                    if (INLINE_SHADOW) {
+                    if (!s.all_opaque) {
                         TraversalCounters ctr = {0, 0};
                         if (!occluded<false>(s, next_origin, light.direction, light.distance, light.triangle_index, rng, st, ctr))
                             L = L + contrib;
@@ -249,33 +300,97 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_shade(DevScene s, PathState ps, 
         }
         const uint32_t ns = wave_append(next_count, cont);
         if (cont) next_queue[ns] = p;
-        if (!INLINE_SHADOW) {
-            const uint32_t ss = wave_append(shadow_count, shadow);
-            if (shadow) shadow_queue[ss] = p;
+        const uint32_t ss = wave_append(shadow_count, shadow);
+        if (shadow) shadow_queue[ss] = p;
+    }
+}
+
+// Per pixel of each unit, add the job's samples in order (trace.slang:1299-1308).
+template <bool MULTI>
+__global__ __launch_bounds__(WAVE_BLOCK) void k_accumulate(WaveJob j0, const WaveJob* __restrict__ jobs, ChunkUnits units,
+                                                           PathState ps) {
+    const uint32_t n = units.n_units * 64u;
+    for (uint32_t t = blockIdx.x * WAVE_BLOCK + threadIdx.x; t < n; t += gridDim.x * WAVE_BLOCK) {
+        const uint32_t u = t >> 6, lane = t & 63u;
+        const WaveJob& jb = job_of<MULTI>(j0, jobs, units.job[u]);
+        uint32_t gx, gy;
+        block_pixel(units.block[u] * 64u + lane, jb.tile_w, gx, gy);
+        if (gx >= jb.tile_w || gy >= jb.tile_h) continue;
+        if (gx + jb.origin_x >= jb.image_w || gy + jb.origin_y >= jb.image_h) continue;
+        const size_t idx = (size_t)gy * jb.tile_w + gx;
+        float4 prev = jb.first_sample > 1u ? jb.accum[idx] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        f3 acc = f3{prev.x, prev.y, prev.z};
+        const uint32_t base = units.offset[u] + lane;
+        for (uint32_t k = 0; k < jb.samples; ++k) acc = acc + ld3(ps.lr, ps.lg, ps.lb, base + k * 64u);
+        jb.accum[idx] = make_float4(acc.x, acc.y, acc.z, 1.0f);
+        if (jb.storage) {
+            f3 o = PT_EXPOSURE * acc / (float)(jb.first_sample + jb.samples - 1u);
+            jb.storage[idx] = make_float4(o.x, o.y, o.z, 1.0f);
         }
     }
 }
 
-// Per pixel, add the chunk's samples in order (trace.slang:1299-1308).
-__global__ __launch_bounds__(WAVE_BLOCK) void k_accumulate(WaveArgs a, PathState ps, float4* __restrict__ accum,
-                                                           float4* __restrict__ storage) {
-    for (uint32_t q = blockIdx.x * WAVE_BLOCK + threadIdx.x; q < a.chunk_pixels; q += gridDim.x * WAVE_BLOCK) {
-        uint32_t gx, gy;
-        block_pixel(a.chunk_first_pixel + q, a.tile.tile_w, gx, gy);
-        if (gx >= a.tile.tile_w || gy >= a.tile.tile_h) continue;
-        if (gx + a.tile.origin_x >= a.tile.image_w || gy + a.tile.origin_y >= a.tile.image_h) continue;
-        const size_t idx = (size_t)gy * a.tile.tile_w + gx;
-        float4 prev = a.tile.first_sample > 1u ? accum[idx] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        f3 acc = f3{prev.x, prev.y, prev.z};
-        for (uint32_t k = 0; k < a.tile.samples; ++k) {
-            const uint32_t p = k * a.chunk_pixels + q;
-            acc = acc + ld3(ps.lr, ps.lg, ps.lb, p);
-        }
-        accum[idx] = make_float4(acc.x, acc.y, acc.z, 1.0f);
-        if (storage) {
-            f3 o = PT_EXPOSURE * acc / (float)(a.tile.first_sample + a.tile.samples - 1u);
-            storage[idx] = make_float4(o.x, o.y, o.z, 1.0f);
-        }
+// ------------------------------------------------------------------ queue sorting
+// Counting sort of a queue by a small key (direction octant with the scene in
+// the high bits, or hit material), so waves see coherent rays / materials.
+template <bool MULTI>
+__device__ __forceinline__ uint32_t sort_key(uint32_t key_kind, const DevScene& s0, const DevScene* scenes,
+                                             const WaveJob* jobs, const PathState& ps, uint32_t p) {
+    if (key_kind == SORT_OCTANT) {
+        uint32_t oct = (ps.dx[p] < 0.0f ? 1u : 0u) | (ps.dy[p] < 0.0f ? 2u : 0u) | (ps.dz[p] < 0.0f ? 4u : 0u);
+        uint32_t sc = MULTI ? (jobs[ps.job[p]].scene & 31u) : 0u;
+        return (sc << 3) | oct;
+    }
+    const int32_t h = ps.hit[p];
+    if (h < 0) return SORT_BUCKETS - 1u;
+    const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
+    return min(s.attr[h].material, SORT_BUCKETS - 2u);
+}
+
+__global__ void k_sort_clear(uint32_t* hist) {
+    if (threadIdx.x < SORT_BUCKETS) hist[threadIdx.x] = 0u;
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(WAVE_BLOCK) void k_sort_hist(uint32_t key_kind, DevScene s0, const DevScene* __restrict__ scenes,
+                                                          const WaveJob* __restrict__ jobs, PathState ps,
+                                                          const uint32_t* __restrict__ queue,
+                                                          const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t local[SORT_BUCKETS];
+    local[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t count = *count_ptr;
+    for (uint32_t j = blockIdx.x * WAVE_BLOCK + threadIdx.x; j < count; j += gridDim.x * WAVE_BLOCK)
+        atomicAdd(&local[sort_key<MULTI>(key_kind, s0, scenes, jobs, ps, queue[j])], 1u);
+    __syncthreads();
+    if (local[threadIdx.x]) atomicAdd(&hist[threadIdx.x], local[threadIdx.x]);
+}
+
+// exclusive scan of the 256 bucket counts in place (one block)
+__global__ __launch_bounds__(SORT_BUCKETS) void k_sort_scan(uint32_t* hist) {
+    __shared__ uint32_t v[SORT_BUCKETS];
+    const uint32_t t = threadIdx.x;
+    v[t] = hist[t];
+    __syncthreads();
+    for (uint32_t off = 1; off < SORT_BUCKETS; off <<= 1) {
+        uint32_t add = t >= off ? v[t - off] : 0u;
+        __syncthreads();
+        v[t] += add;
+        __syncthreads();
+    }
+    hist[t] = v[t] - hist[t];
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(WAVE_BLOCK) void k_sort_scatter(uint32_t key_kind, DevScene s0, const DevScene* __restrict__ scenes,
+                                                             const WaveJob* __restrict__ jobs, PathState ps,
+                                                             const uint32_t* __restrict__ queue,
+                                                             const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ hist,
+                                                             uint32_t* __restrict__ out) {
+    const uint32_t count = *count_ptr;
+    for (uint32_t j = blockIdx.x * WAVE_BLOCK + threadIdx.x; j < count; j += gridDim.x * WAVE_BLOCK) {
+        const uint32_t p = queue[j];
+        out[atomicAdd(&hist[sort_key<MULTI>(key_kind, s0, scenes, jobs, ps, p)], 1u)] = p;
     }
 }
 
@@ -286,24 +401,174 @@ namespace ptcore {
 
 using namespace ptd;
 
+namespace {
+
+// Records a start/stop event pair around a kernel on the stream it runs on;
+// no synchronisation until the log is read.
+struct EventScope {
+    EventLog* log;
+    int kind;
+    hipStream_t st;
+    hipEvent_t a = nullptr, b = nullptr;
+    EventScope(EventLog* l, int k, hipStream_t s) : log(l), kind(k), st(s) {
+        if (log) {
+            log->take(a, b);
+            hipEventRecord(a, st);
+        }
+    }
+    ~EventScope() {
+        if (log) {
+            hipEventRecord(b, st);
+            log->pairs.push_back({kind, a, b});
+        }
+    }
+};
+
+template <bool COUNT, bool MULTI>
+void launch_extend(uint32_t lds, uint32_t grid, hipStream_t st, const DevScene& s0, const DevScene* scenes,
+                   const WaveJob* jobs, const PathState& ps, const uint32_t* q, const uint32_t* cnt, uint32_t* ovf,
+                   unsigned long long* ctr) {
+    switch (lds) {
+        case 24: k_extend<24, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, q, cnt, ovf, ctr); break;
+        case 32: k_extend<32, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, q, cnt, ovf, ctr); break;
+        default: k_extend<16, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, q, cnt, ovf, ctr); break;
+    }
+}
+template <bool COUNT, bool MULTI>
+void launch_shadow(uint32_t lds, uint32_t grid, hipStream_t st, const DevScene& s0, const DevScene* scenes,
+                   const WaveJob* jobs, const PathState& ps, const uint32_t* q, const uint32_t* cnt, uint32_t* ovf,
+                   unsigned long long* ctr) {
+    switch (lds) {
+        case 24: k_shadow<24, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, q, cnt, ovf, ctr); break;
+        case 32: k_shadow<32, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, q, cnt, ovf, ctr); break;
+        default: k_shadow<16, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, q, cnt, ovf, ctr); break;
+    }
+}
+
+struct Launch {
+    uint32_t grid;
+    hipStream_t stream;
+    const DevScene* s0;
+    const DevScene* scenes;
+    const WaveJob* jobs;
+    const WaveJob* j0;
+    ChunkUnits units;
+};
+
+template <bool MULTI>
+void run_chunk(Wavefront& w, const Launch& L, uint32_t max_bounces, bool counting, EventLog* lg,
+               unsigned long long* ctr) {
+    const hipStream_t stream = L.stream;
+    const uint32_t grid = L.grid;
+    const PathState& ps = w.ps;
+    k_zero_counts<<<1, 64, 0, stream>>>(w.counts);
+    const uint32_t gen_grid = std::min<uint32_t>(grid, (L.units.n_paths + WAVE_BLOCK - 1) / WAVE_BLOCK);
+    k_generate<MULTI><<<gen_grid, WAVE_BLOCK, 0, stream>>>(*L.s0, *L.j0, L.scenes, L.jobs, L.units, ps, w.queue[0], w.counts);
+    uint32_t* cur = w.queue[0];
+    uint32_t* nxt = w.queue[1];
+    uint32_t* spare = w.queue[3];
+    for (uint32_t b = 0; b <= max_bounces; ++b) {
+        uint32_t* cnt = w.counts + 2 * b;
+        if (w.cfg.sort_key == SORT_OCTANT && b > 0) {
+            k_sort_clear<<<1, SORT_BUCKETS, 0, stream>>>(w.sort_hist);
+            k_sort_hist<MULTI><<<grid, WAVE_BLOCK, 0, stream>>>(SORT_OCTANT, *L.s0, L.scenes, L.jobs, ps, cur, cnt, w.sort_hist);
+            k_sort_scan<<<1, SORT_BUCKETS, 0, stream>>>(w.sort_hist);
+            k_sort_scatter<MULTI><<<grid, WAVE_BLOCK, 0, stream>>>(SORT_OCTANT, *L.s0, L.scenes, L.jobs, ps, cur, cnt,
+                                                                    w.sort_hist, spare);
+            std::swap(cur, spare);
+        }
+        {
+            EventScope tm(lg, EV_EXTEND, stream);
+            if (counting)
+                launch_extend<true, MULTI>(w.cfg.lds_depth, grid, stream, *L.s0, L.scenes, L.jobs, ps, cur, cnt, w.ovf, ctr);
+            else
+                launch_extend<false, MULTI>(w.cfg.lds_depth, grid, stream, *L.s0, L.scenes, L.jobs, ps, cur, cnt, w.ovf, ctr);
+        }
+        if (w.cfg.sort_key == SORT_MATERIAL) {
+            k_sort_clear<<<1, SORT_BUCKETS, 0, stream>>>(w.sort_hist);
+            k_sort_hist<MULTI><<<grid, WAVE_BLOCK, 0, stream>>>(SORT_MATERIAL, *L.s0, L.scenes, L.jobs, ps, cur, cnt, w.sort_hist);
+            k_sort_scan<<<1, SORT_BUCKETS, 0, stream>>>(w.sort_hist);
+            k_sort_scatter<MULTI><<<grid, WAVE_BLOCK, 0, stream>>>(SORT_MATERIAL, *L.s0, L.scenes, L.jobs, ps, cur, cnt,
+                                                                    w.sort_hist, spare);
+            std::swap(cur, spare);
+        }
+        {
+            EventScope tm(lg, EV_SHADE, stream);
+            k_shade<MULTI><<<grid, WAVE_BLOCK, 0, stream>>>(*L.s0, L.scenes, L.jobs, *L.j0, ps, b, cur, cnt, nxt,
+                                                            w.counts + 2 * (b + 1), w.queue[2], cnt + 1, w.ovf);
+        }
+        {
+            EventScope tm(lg, EV_SHADOW, stream);
+            if (counting)
+                launch_shadow<true, MULTI>(w.cfg.lds_depth, grid, stream, *L.s0, L.scenes, L.jobs, ps, w.queue[2], cnt + 1, w.ovf, ctr);
+            else
+                launch_shadow<false, MULTI>(w.cfg.lds_depth, grid, stream, *L.s0, L.scenes, L.jobs, ps, w.queue[2], cnt + 1, w.ovf, ctr);
+        }
+        std::swap(cur, nxt);
+        // keep `spare` distinct from the live queues
+        if (spare == cur || spare == nxt) spare = (w.queue[0] != cur && w.queue[0] != nxt) ? w.queue[0]
+                                                 : (w.queue[1] != cur && w.queue[1] != nxt) ? w.queue[1] : w.queue[3];
+    }
+    const uint32_t acc_grid = std::min<uint32_t>(grid, (L.units.n_units * 64u + WAVE_BLOCK - 1) / WAVE_BLOCK);
+    k_accumulate<MULTI><<<acc_grid, WAVE_BLOCK, 0, stream>>>(*L.j0, L.jobs, L.units, ps);
+}
+
+}  // namespace
+
+void EventLog::take(hipEvent_t& a, hipEvent_t& b) {
+    if (pool.size() < 2) {
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        return;
+    }
+    a = pool.back();
+    pool.pop_back();
+    b = pool.back();
+    pool.pop_back();
+}
+void EventLog::reset() {
+    for (auto& p : pairs) {
+        pool.push_back(p.a);
+        pool.push_back(p.b);
+    }
+    pairs.clear();
+}
+void EventLog::sum(float ms[EV_KINDS], uint32_t launches[EV_KINDS]) {
+    for (int k = 0; k < EV_KINDS; ++k) ms[k] = 0.0f, launches[k] = 0;
+    if (!pairs.empty()) hipEventSynchronize(pairs.back().b);
+    for (auto& p : pairs) {
+        float t = 0.0f;
+        hipEventElapsedTime(&t, p.a, p.b);
+        ms[p.kind] += t;
+        launches[p.kind] += 1;
+    }
+}
+EventLog::~EventLog() {
+    reset();
+    for (hipEvent_t e : pool) hipEventDestroy(e);
+}
+
 Wavefront::~Wavefront() {
     if (arena) hipFree(arena);
     if (ovf) hipFree(ovf);
+    if (tables) hipFree(tables);
+    if (staging) hipHostFree(staging);
 }
 
 pt_status Wavefront::reserve(uint32_t paths) {
     if (paths <= capacity && arena) return PT_OK;
     if (arena) {
-        hipStreamSynchronize(ctx->stream);
+        hipStreamSynchronize(stream_override ? stream_override : ctx->stream);
         hipFree(arena);
         arena = nullptr;
+        capacity = 0;
     }
-    // 29 per-path 4-byte fields + 3 queues, each 256-byte aligned, + counters
+    // PATH_FIELDS per-path 4-byte arrays + 4 queues, each 256-byte aligned, + counters + sort histogram
     const size_t field_bytes = ((size_t)paths * 4 + 255) & ~(size_t)255;
-    arena_bytes = 32 * field_bytes + ((MAX_COUNTS * 4 + 255) & ~(size_t)255);
+    arena_bytes = (PATH_FIELDS + 4) * field_bytes + ((MAX_COUNTS * 4 + 255) & ~(size_t)255) + SORT_BUCKETS * 4;
     hipError_t e = hipMalloc(&arena, arena_bytes);
     if (e != hipSuccess) {
-        capacity = 0;
+        arena = nullptr;
         return hip_error(e, "wavefront path state");
     }
     capacity = paths;
@@ -313,83 +578,51 @@ pt_status Wavefront::reserve(uint32_t paths) {
         p += (bytes + 255) & ~(size_t)255;
         return r;
     };
-    const size_t f = (size_t)paths * 4;
     float** fields[] = {&ps.ox, &ps.oy, &ps.oz, &ps.dx, &ps.dy, &ps.dz, &ps.tr, &ps.tg, &ps.tb, &ps.lr, &ps.lg, &ps.lb,
                         &ps.prev_pdf, &ps.ht, &ps.hv, &ps.hw, &ps.sox, &ps.soy, &ps.soz, &ps.sdx, &ps.sdy, &ps.sdz,
                         &ps.sdist, &ps.scr, &ps.scg, &ps.scb};
-    for (float** fp : fields) *fp = (float*)take(f);
-    ps.rng = (uint32_t*)take(f);
-    ps.hit = (int32_t*)take(f);
-    ps.slight = (int32_t*)take(f);
-    for (auto& q : queue) q = (uint32_t*)take(f);
+    for (float** fp : fields) *fp = (float*)take(field_bytes);
+    ps.rng = (uint32_t*)take(field_bytes);
+    ps.job = (uint32_t*)take(field_bytes);
+    ps.hit = (int32_t*)take(field_bytes);
+    ps.slight = (int32_t*)take(field_bytes);
+    static_assert(sizeof(fields) / sizeof(fields[0]) + 4 == PATH_FIELDS, "PathState field count");
+    for (auto& q : queue) q = (uint32_t*)take(field_bytes);
     counts = (uint32_t*)take(MAX_COUNTS * 4);
+    sort_hist = (uint32_t*)take(SORT_BUCKETS * 4);
     if ((size_t)(p - (char*)arena) > arena_bytes) return set_error(PT_ERR_STATE, "path arena overrun");
     return PT_OK;
 }
 
-namespace {
-
-template <bool COUNT>
-void launch_extend(uint32_t lds_depth, uint32_t grid, hipStream_t st, const DevScene& s, const PathState& ps,
-                   const uint32_t* q, const uint32_t* cnt, uint32_t* ovf, unsigned long long* ctr) {
-    switch (lds_depth) {
-        case 24: k_extend<24, COUNT><<<grid, WAVE_BLOCK, 0, st>>>(s, ps, q, cnt, ovf, ctr); break;
-        case 32: k_extend<32, COUNT><<<grid, WAVE_BLOCK, 0, st>>>(s, ps, q, cnt, ovf, ctr); break;
-        default: k_extend<16, COUNT><<<grid, WAVE_BLOCK, 0, st>>>(s, ps, q, cnt, ovf, ctr); break;
+pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, EventLog* log,
+                            unsigned long long* host_counters) {
+    if (jobs.empty()) return PT_OK;
+    cfg = ctx->wave_cfg;
+    const hipStream_t stream = stream_override ? stream_override : ctx->stream;
+    const bool multi = jobs.size() > 1;
+    // work units: (job, 64-pixel block), all of the job's samples
+    uint32_t max_unit_paths = 0, max_bounces = 0, max_depth = 1;
+    bool any_alpha = false;
+    size_t total_units = 0;
+    for (const HostJob& h : jobs) {
+        const WaveJob& j = h.job;
+        if (j.samples == 0 || j.tile_w == 0 || j.tile_h == 0) return set_error(PT_ERR_INVALID, "empty job");
+        if ((uint64_t)j.samples * 64u > (1u << 30)) return set_error(PT_ERR_INVALID, "too many samples in one job");
+        max_unit_paths = std::max(max_unit_paths, 64u * j.samples);
+        max_bounces = std::max(max_bounces, j.max_bounces);
+        max_depth = std::max(max_depth, h.scene->bvh_depth + 1u);
+        any_alpha = any_alpha || !h.scene->view.all_opaque;
+        const uint32_t be = h.block_end ? h.block_end : tile_blocks(j);
+        if (h.block_begin > be || be > tile_blocks(j)) return set_error(PT_ERR_INVALID, "job block range out of the tile");
+        total_units += be - h.block_begin;
     }
-}
-template <bool COUNT>
-void launch_shadow(uint32_t lds_depth, uint32_t grid, hipStream_t st, const DevScene& s, const PathState& ps,
-                   const uint32_t* q, const uint32_t* cnt, uint32_t* ovf, unsigned long long* ctr) {
-    switch (lds_depth) {
-        case 24: k_shadow<24, COUNT><<<grid, WAVE_BLOCK, 0, st>>>(s, ps, q, cnt, ovf, ctr); break;
-        case 32: k_shadow<32, COUNT><<<grid, WAVE_BLOCK, 0, st>>>(s, ps, q, cnt, ovf, ctr); break;
-        default: k_shadow<16, COUNT><<<grid, WAVE_BLOCK, 0, st>>>(s, ps, q, cnt, ovf, ctr); break;
-    }
-}
-
-struct EventTimer {
-    bool on;
-    hipStream_t st;
-    hipEvent_t a = nullptr, b = nullptr;
-    float* sink;
-    EventTimer(bool enabled, hipStream_t s, float* out) : on(enabled), st(s), sink(out) {
-        if (on) {
-            hipEventCreate(&a);
-            hipEventCreate(&b);
-            hipEventRecord(a, st);
-        }
-    }
-    ~EventTimer() {
-        if (on) {
-            hipEventRecord(b, st);
-            hipEventSynchronize(b);
-            float ms = 0.0f;
-            hipEventElapsedTime(&ms, a, b);
-            *sink += ms;
-            hipEventDestroy(a);
-            hipEventDestroy(b);
-        }
-    }
-};
-
-}  // namespace
-
-pt_status Wavefront::render(Scene& scene, const TileArgs& tile, float4* accum, float4* storage, int profiling,
-                            pt_trace_stats* stats) {
-    const hipStream_t stream = ctx->stream;
-    const uint32_t bw = (tile.tile_w + 7u) / 8u, bh = (tile.tile_h + 7u) / 8u;
-    const uint32_t tile_pixels = bw * bh * 64u;
-    const uint32_t S = tile.samples;
-    uint32_t chunk = std::max<uint32_t>(64u, (max_chunk_paths / S) & ~63u);
-    chunk = std::min(chunk, tile_pixels);
-    pt_status st = reserve(chunk * S);
+    (void)any_alpha;
+    const uint32_t chunk_cap = std::max(cfg.chunk_paths, max_unit_paths);
+    pt_status st = reserve(chunk_cap);
     if (st) return st;
     const uint32_t grid = ctx->cu_count * 8u;
-    const uint32_t depth = scene.bvh_depth + 1u;
-    const uint32_t lds = extend_cfg.lds_depth;
-    const size_t need = (size_t)(depth > std::min(lds, MEGA_LDS_DEPTH) ? depth - std::min(lds, MEGA_LDS_DEPTH) : 1u) *
-                        grid * WAVE_BLOCK * 4u;
+    const uint32_t lds_min = std::min(cfg.lds_depth, MEGA_LDS_DEPTH);
+    const size_t need = (size_t)(max_depth > lds_min ? max_depth - lds_min : 1u) * grid * WAVE_BLOCK * 4u;
     if (need > ovf_bytes) {
         if (ovf) {
             hipStreamSynchronize(stream);
@@ -403,77 +636,122 @@ pt_status Wavefront::render(Scene& scene, const TileArgs& tile, float4* accum, f
         }
         ovf_bytes = need;
     }
-    const DevScene& s = scene.view;
-    const bool inline_shadow = !s.all_opaque;
-    if (profiling) {
-        std::memset(stats, 0, sizeof(*stats));
-        hipMemsetAsync(ctx->counters, 0, 16 * sizeof(unsigned long long), stream);
+    // host tables: scenes, jobs, then the units of every chunk (job, block, offset)
+    std::vector<DevScene> scene_tab;
+    std::vector<WaveJob> job_tab(jobs.size());
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        const Scene* sc = jobs[i].scene;
+        uint32_t idx = 0;
+        for (; idx < scene_tab.size(); ++idx)
+            if (std::memcmp(&scene_tab[idx], &sc->view, sizeof(DevScene)) == 0) break;
+        if (idx == scene_tab.size()) scene_tab.push_back(sc->view);
+        job_tab[i] = jobs[i].job;
+        job_tab[i].scene = idx;
     }
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    if (profiling) {
-        hipEventCreate(&t0);
-        hipEventCreate(&t1);
-        hipEventRecord(t0, stream);
-    }
-    unsigned long long* ctr = profiling ? ctx->counters : nullptr;
-    for (uint32_t first = 0; first < tile_pixels; first += chunk) {
-        WaveArgs a;
-        a.tile = tile;
-        a.chunk_first_pixel = first;
-        a.chunk_pixels = std::min(chunk, tile_pixels - first);
-        a.tile_pixels = tile_pixels;
-        k_zero_counts<<<1, 64, 0, stream>>>(counts);
-        const uint32_t gen_grid = std::min<uint32_t>(grid, (a.chunk_pixels * S + WAVE_BLOCK - 1) / WAVE_BLOCK);
-        k_generate<<<gen_grid, WAVE_BLOCK, 0, stream>>>(s, a, ps, queue[0], counts);
-        for (uint32_t b = 0; b <= tile.max_bounces; ++b) {
-            uint32_t* q = queue[b & 1u];
-            uint32_t* qn = queue[(b + 1u) & 1u];
-            {
-                EventTimer tm(profiling, stream, &stats->extend_ms);
-                if (profiling) launch_extend<true>(lds, grid, stream, s, ps, q, counts + 2 * b, ovf, ctr);
-                else launch_extend<false>(lds, grid, stream, s, ps, q, counts + 2 * b, ovf, ctr);
+    struct ChunkDesc {
+        size_t unit_begin, unit_end;
+        uint32_t paths;
+    };
+    std::vector<ChunkDesc> chunks;
+    std::vector<uint32_t> ujob, ublock, uoff;
+    ujob.reserve(total_units);
+    ublock.reserve(total_units);
+    uoff.reserve(total_units + 64);
+    {
+        ChunkDesc cur{0, 0, 0};
+        for (size_t ji = 0; ji < jobs.size(); ++ji) {
+            const WaveJob& j = job_tab[ji];
+            const uint32_t be = jobs[ji].block_end ? jobs[ji].block_end : tile_blocks(j);
+            const uint32_t up = 64u * j.samples;
+            for (uint32_t b = jobs[ji].block_begin; b < be; ++b) {
+                if (cur.paths + up > chunk_cap) {
+                    cur.unit_end = ujob.size();
+                    chunks.push_back(cur);
+                    cur = ChunkDesc{ujob.size(), 0, 0};
+                }
+                ujob.push_back((uint32_t)ji);
+                ublock.push_back(b);
+                uoff.push_back(cur.paths);
+                cur.paths += up;
             }
-            {
-                EventTimer tm(profiling, stream, &stats->shade_ms);
-                if (inline_shadow)
-                    k_shade<true><<<grid, WAVE_BLOCK, 0, stream>>>(s, ps, b, tile.max_bounces, q, counts + 2 * b, qn,
-                                                                  counts + 2 * (b + 1), queue[2], counts + 2 * b + 1, ovf);
-                else
-                    k_shade<false><<<grid, WAVE_BLOCK, 0, stream>>>(s, ps, b, tile.max_bounces, q, counts + 2 * b, qn,
-                                                                   counts + 2 * (b + 1), queue[2], counts + 2 * b + 1, ovf);
-            }
-            if (!inline_shadow) {
-                EventTimer tm(profiling, stream, &stats->shadow_ms);
-                if (profiling) launch_shadow<true>(lds, grid, stream, s, ps, queue[2], counts + 2 * b + 1, ovf, ctr);
-                else launch_shadow<false>(lds, grid, stream, s, ps, queue[2], counts + 2 * b + 1, ovf, ctr);
-            }
-            if (profiling) stats->extend_launches += 1;
         }
-        const uint32_t acc_grid = std::min<uint32_t>(grid, (a.chunk_pixels + WAVE_BLOCK - 1) / WAVE_BLOCK);
-        k_accumulate<<<acc_grid, WAVE_BLOCK, 0, stream>>>(a, ps, accum, storage);
-        hipError_t e = hipGetLastError();
+        cur.unit_end = ujob.size();
+        if (cur.paths) chunks.push_back(cur);
+    }
+    // offsets array per chunk needs its terminating total: lay out as
+    // [units of chunk 0 offsets..., total0, units of chunk 1 offsets..., total1, ...]
+    std::vector<uint32_t> off_packed;
+    off_packed.reserve(uoff.size() + chunks.size());
+    std::vector<size_t> off_start(chunks.size());
+    for (size_t c = 0; c < chunks.size(); ++c) {
+        off_start[c] = off_packed.size();
+        for (size_t u = chunks[c].unit_begin; u < chunks[c].unit_end; ++u) off_packed.push_back(uoff[u]);
+        off_packed.push_back(chunks[c].paths);
+    }
+    const size_t b_scenes = scene_tab.size() * sizeof(DevScene), b_jobs = job_tab.size() * sizeof(WaveJob);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t bytes = al(b_scenes) + al(b_jobs) + al(ujob.size() * 4) + al(ublock.size() * 4) + al(off_packed.size() * 4);
+    if (bytes > tables_bytes) {
+        hipStreamSynchronize(stream);
+        if (tables) hipFree(tables);
+        if (staging) hipHostFree(staging);
+        tables = staging = nullptr;
+        tables_bytes = staging_bytes = 0;
+        hipError_t e = hipMalloc(&tables, bytes);
+        if (e == hipSuccess) e = hipHostMalloc(&staging, bytes, 0);
+        if (e != hipSuccess) return hip_error(e, "wavefront job tables");
+        tables_bytes = staging_bytes = bytes;
+    } else {
+        hipStreamSynchronize(stream);  // staging may still feed an earlier copy
+    }
+    char* hs = (char*)staging;
+    char* ds = (char*)tables;
+    size_t o = 0;
+    auto put = [&](const void* src, size_t n) {
+        std::memcpy(hs + o, src, n);
+        char* d = ds + o;
+        o += al(n);
+        return d;
+    };
+    const DevScene* d_scenes = (const DevScene*)put(scene_tab.data(), b_scenes);
+    const WaveJob* d_jobs = (const WaveJob*)put(job_tab.data(), b_jobs);
+    const uint32_t* d_ujob = (const uint32_t*)put(ujob.data(), ujob.size() * 4);
+    const uint32_t* d_ublock = (const uint32_t*)put(ublock.data(), ublock.size() * 4);
+    const uint32_t* d_uoff = (const uint32_t*)put(off_packed.data(), off_packed.size() * 4);
+    hipError_t e = hipMemcpyAsync(tables, staging, o, hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return hip_error(e, "job table upload");
+
+    const bool counting = profiling >= 2;
+    EventLog* lg = profiling >= 1 ? log : nullptr;
+    unsigned long long* ctr = counting ? ctx->counters : nullptr;
+    if (counting) hipMemsetAsync(ctx->counters, 0, 16 * sizeof(unsigned long long), stream);
+    EventScope total(lg, EV_TOTAL, stream);
+    for (size_t c = 0; c < chunks.size(); ++c) {
+        Launch L;
+        L.grid = grid;
+        L.stream = stream;
+        L.s0 = &scene_tab[0];
+        L.j0 = &job_tab[0];
+        L.scenes = d_scenes;
+        L.jobs = d_jobs;
+        L.units.job = d_ujob + chunks[c].unit_begin;
+        L.units.block = d_ublock + chunks[c].unit_begin;
+        L.units.offset = d_uoff + off_start[c];
+        L.units.n_units = (uint32_t)(chunks[c].unit_end - chunks[c].unit_begin);
+        L.units.n_paths = chunks[c].paths;
+        if (multi) run_chunk<true>(*this, L, max_bounces, counting, lg, ctr);
+        else run_chunk<false>(*this, L, max_bounces, counting, lg, ctr);
+        e = hipGetLastError();
         if (e != hipSuccess) return hip_error(e, "wavefront launch");
-        if (profiling) {
+        if (counting) {
             uint32_t h[MAX_COUNTS];
             hipMemcpyAsync(h, counts, sizeof(h), hipMemcpyDeviceToHost, stream);
             hipStreamSynchronize(stream);
-            for (uint32_t b = 0; b <= tile.max_bounces; ++b) {
-                stats->closest_rays += h[2 * b];
-                stats->shadow_rays += h[2 * b + 1];
+            for (uint32_t b = 0; b <= max_bounces; ++b) {
+                host_counters[0] += h[2 * b];
+                host_counters[1] += h[2 * b + 1];
             }
         }
-    }
-    if (profiling) {
-        hipEventRecord(t1, stream);
-        hipEventSynchronize(t1);
-        hipEventElapsedTime(&stats->total_ms, t0, t1);
-        hipEventDestroy(t0);
-        hipEventDestroy(t1);
-        unsigned long long h[4];
-        hipMemcpy(h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost);
-        stats->pair_tests = h[2];
-        stats->tri_tests = h[3];
-        if (inline_shadow) stats->shadow_rays = 0;  // counted inline, not queued
     }
     return PT_OK;
 }

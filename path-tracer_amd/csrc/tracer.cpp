@@ -11,23 +11,39 @@
 #include <cstring>
 
 #include "pt_runtime.h"
-#include "wavefront.h"
+#include "tracer.h"
 
 using namespace ptcore;
 
-struct pt_tracer {
-    Context* ctx = nullptr;
-    Scene* scene = nullptr;
-    uint64_t scene_version = 0;
-    uint32_t tile_w = 0, tile_h = 0;
-    PtTraceParameters params{};
-    int schedule = PT_SCHEDULE_WAVEFRONT;
-    float4* accum = nullptr;
-    float4* storage = nullptr;
-    int profiling = 0;
-    pt_trace_stats stats{};
-    Wavefront* wave = nullptr;
-};
+namespace ptcore {
+
+void tracer_sync_version(pt_tracer* t) {
+    if (t->scene->version != t->scene_version) {
+        t->scene_version = t->scene->version;
+        t->params.sample_count = 0;  // reset_accumulation (path_tracer.rs:278-280)
+    }
+}
+
+HostJob tracer_job(pt_tracer* t, uint32_t samples) {
+    HostJob hj;
+    hj.scene = t->scene;
+    ptd::WaveJob& j = hj.job;
+    j.scene = 0;
+    j.image_w = t->params.image_size[0];
+    j.image_h = t->params.image_size[1];
+    j.origin_x = t->params.tile_origin[0];
+    j.origin_y = t->params.tile_origin[1];
+    j.tile_w = t->tile_w;
+    j.tile_h = t->tile_h;
+    j.max_bounces = t->params.max_bounces;
+    j.first_sample = t->params.sample_count + 1u;
+    j.samples = samples;
+    j.accum = t->accum;
+    j.storage = t->storage;
+    return hj;
+}
+
+}  // namespace ptcore
 
 namespace {
 
@@ -54,12 +70,7 @@ pt_status validate_tile(uint32_t w, uint32_t h) {
     return PT_OK;
 }
 
-void sync_version(pt_tracer* t) {
-    if (t->scene->version != t->scene_version) {
-        t->scene_version = t->scene->version;
-        t->params.sample_count = 0;  // reset_accumulation (path_tracer.rs:278-280)
-    }
-}
+void sync_version(pt_tracer* t) { tracer_sync_version(t); }
 
 ptd::TileArgs tile_args(const pt_tracer* t, uint32_t first_sample, uint32_t samples) {
     ptd::TileArgs a;
@@ -87,38 +98,38 @@ pt_status trace_samples(pt_tracer* t, uint32_t samples) {
         const uint32_t blocks = ((t->tile_w + 15u) / 16u) * ((t->tile_h + 15u) / 16u);
         const uint32_t grid = std::min<uint32_t>(blocks, c->cu_count * 8u);
         if ((st = c->ensure_overflow(t->scene->bvh_depth + 1u, grid * 256u))) return st;
+        const bool counting = t->profiling >= 2;
+        if (counting) hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream);
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (t->profiling) {
-            hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream);
-            hipEventCreate(&e0);
-            hipEventCreate(&e1);
+            t->log.take(e0, e1);
             hipEventRecord(e0, c->stream);
         }
         hipError_t e = launch_megakernel(t->scene->view, a, t->accum, t->storage, c->ovf, grid,
-                                         t->profiling ? c->counters : nullptr, c->stream);
+                                         counting ? c->counters : nullptr, c->stream);
         if (e != hipSuccess) return hip_error(e, "megakernel launch");
         if (t->profiling) {
             hipEventRecord(e1, c->stream);
-            hipEventSynchronize(e1);
+            t->log.pairs.push_back({EV_TOTAL, e0, e1});
+        }
+        if (counting) {
             unsigned long long h[4];
-            hipMemcpy(h, c->counters, sizeof(h), hipMemcpyDeviceToHost);
-            float ms = 0.0f;
-            hipEventElapsedTime(&ms, e0, e1);
-            hipEventDestroy(e0);
-            hipEventDestroy(e1);
-            std::memset(&t->stats, 0, sizeof(t->stats));
-            t->stats.closest_rays = h[0];
-            t->stats.shadow_rays = h[1];
-            t->stats.pair_tests = h[2];
-            t->stats.tri_tests = h[3];
-            t->stats.total_ms = ms;
+            hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+            hipStreamSynchronize(c->stream);
+            for (int k = 0; k < 4; ++k) t->counters[k] += h[k];
         }
     } else {
         if (!t->wave) {
             t->wave = new Wavefront();
             t->wave->ctx = c;
         }
-        if ((st = t->wave->render(*t->scene, a, t->accum, t->storage, t->profiling, &t->stats))) return st;
+        if ((st = t->wave->render({tracer_job(t, samples)}, t->profiling, &t->log, t->counters))) return st;
+        if (t->profiling >= 2) {
+            unsigned long long h[6];
+            hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+            hipStreamSynchronize(c->stream);
+            for (int k = 2; k < 6; ++k) t->counters[k] += h[k];
+        }
     }
     t->params.sample_count += samples;
     return PT_OK;
@@ -287,15 +298,40 @@ pt_status pt_tracer_primary_hits(pt_tracer* t, uint32_t sample_count, int32_t* t
     return e == hipSuccess ? PT_OK : hip_error(e, "primary hits");
 }
 
-pt_status pt_tracer_set_profiling(pt_tracer* t, int enabled) {
+pt_status pt_tracer_set_profiling(pt_tracer* t, int level) {
     if (!t) return set_error(PT_ERR_INVALID, "null tracer");
-    t->profiling = enabled ? 1 : 0;
+    if (level < 0 || level > 2) return set_error(PT_ERR_INVALID, "profiling level %d not in 0..2", level);
+    hipSetDevice(t->ctx->device);
+    hipStreamSynchronize(t->ctx->stream);
+    t->profiling = level;
+    t->log.reset();
+    std::memset(t->counters, 0, sizeof(t->counters));
     return PT_OK;
 }
 
 pt_status pt_tracer_stats(pt_tracer* t, pt_trace_stats* out) {
     if (!t || !out) return set_error(PT_ERR_INVALID, "null argument");
-    *out = t->stats;
+    hipSetDevice(t->ctx->device);
+    float ms[EV_KINDS];
+    uint32_t launches[EV_KINDS];
+    t->log.sum(ms, launches);
+    std::memset(out, 0, sizeof(*out));
+    out->closest_rays = t->counters[0];
+    out->shadow_rays = t->counters[1];
+    if (t->schedule == PT_SCHEDULE_MEGAKERNEL) {
+        out->pair_tests = t->counters[2];
+        out->tri_tests = t->counters[3];
+    } else {
+        out->pair_tests = t->counters[2];  // closest-hit traversal (extend)
+        out->tri_tests = t->counters[3];
+        out->shadow_pair_tests = t->counters[4];
+        out->shadow_tri_tests = t->counters[5];
+    }
+    out->extend_launches = launches[EV_EXTEND];
+    out->extend_ms = ms[EV_EXTEND];
+    out->shade_ms = ms[EV_SHADE];
+    out->shadow_ms = ms[EV_SHADOW];
+    out->total_ms = ms[EV_TOTAL];
     return PT_OK;
 }
 

@@ -122,9 +122,40 @@ class WorldDesc(C.Structure):
 
 class TraceStats(C.Structure):
     _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("pair_tests", C.c_uint64),
-                ("tri_tests", C.c_uint64), ("extend_launches", C.c_uint64), ("extend_ms", C.c_float),
+                ("tri_tests", C.c_uint64), ("shadow_pair_tests", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
+                ("extend_launches", C.c_uint64), ("extend_ms", C.c_float),
                 ("shadow_ms", C.c_float), ("shade_ms", C.c_float), ("total_ms", C.c_float)]
 
+
+class WaveConfig(C.Structure):
+    _fields_ = [("lds_stack_depth", C.c_uint32), ("chunk_paths", C.c_uint32), ("sort_key", C.c_uint32)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("best", WaveConfig), ("best_ms", C.c_float), ("initial_ms", C.c_float), ("trials", C.c_uint32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active_jobs", C.c_uint32), ("round_paths", C.c_uint32)]
+
+
+class RenderRequest(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("samples", C.c_uint32),
+                ("max_bounces", C.c_uint32)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("rounds", C.c_uint32), ("submit_s", C.c_double), ("admit_s", C.c_double),
+                ("done_s", C.c_double), ("footprint_bytes", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("rounds", C.c_uint64), ("paths", C.c_uint64), ("jobs_done", C.c_uint64),
+                ("jobs_rejected", C.c_uint64), ("busy_s", C.c_double), ("bytes_in_use", C.c_uint64)]
+
+
+SORT_NONE, SORT_OCTANT, SORT_MATERIAL = 0, 1, 2
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
 
 # every symbol include/pt_api.h and include/pt_scenes.h declare: (name, restype, argtypes)
 _u32, _i32, _f32 = C.c_uint32, C.c_int32, C.c_float
@@ -162,6 +193,18 @@ _SIGNATURES = [
     ("pt_tracer_primary_hits", _i32, [_p, _u32, _p, _p]),
     ("pt_tracer_set_profiling", _i32, [_p, C.c_int]),
     ("pt_tracer_stats", _i32, [_p, C.POINTER(TraceStats)]),
+    ("pt_context_set_wave_config", _i32, [_p, C.POINTER(WaveConfig)]),
+    ("pt_context_get_wave_config", _i32, [_p, C.POINTER(WaveConfig)]),
+    ("pt_trace_batch", _i32, [C.POINTER(_p), _u32, C.POINTER(_u32)]),
+    ("pt_autotune_batch", _i32, [C.POINTER(_p), _u32, C.POINTER(_u32), C.POINTER(TuneResult)]),
+    ("pt_server_create", _i32, [_p, C.POINTER(ServerConfig), C.POINTER(_p)]),
+    ("pt_server_destroy", _i32, [_p]),
+    ("pt_server_submit", _i32, [_p, C.POINTER(RenderRequest), C.POINTER(C.c_uint64)]),
+    ("pt_server_wait", _i32, [_p, C.c_uint64, _u32]),
+    ("pt_server_info", _i32, [_p, C.c_uint64, C.POINTER(JobInfo)]),
+    ("pt_server_result", _i32, [_p, C.c_uint64, _p]),
+    ("pt_server_cancel", _i32, [_p, C.c_uint64]),
+    ("pt_server_stats_get", _i32, [_p, C.POINTER(ServerStats)]),
     ("pt_generate_scene", _i32, [C.c_int, C.c_uint64, _u32, _u32, _u32, C.POINTER(_p)]),
     ("pt_generated_world_desc", _i32, [_p, C.POINTER(WorldDesc)]),
     ("pt_generated_destroy", _i32, [_p]),
@@ -546,8 +589,9 @@ class Tracer:
         _check(lib().pt_tracer_primary_hits(self.handle, sample_count, _ptr(tri), _ptr(t)))
         return tri, t
 
-    def set_profiling(self, on=True):
-        _check(lib().pt_tracer_set_profiling(self.handle, 1 if on else 0))
+    def set_profiling(self, level=1):
+        """0 off, 1 per-kernel HIP-event timing, 2 timing + traversal counters."""
+        _check(lib().pt_tracer_set_profiling(self.handle, int(level)))
 
     def stats(self):
         s = TraceStats()
@@ -557,6 +601,88 @@ class Tracer:
     def close(self):
         if self.handle:
             lib().pt_tracer_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def set_wave_config(ctx: Context, lds_stack_depth=16, chunk_paths=1 << 22, sort_key=SORT_NONE):
+    cfg = WaveConfig(lds_stack_depth, chunk_paths, sort_key)
+    _check(lib().pt_context_set_wave_config(ctx.handle, C.byref(cfg)))
+
+
+def get_wave_config(ctx: Context):
+    cfg = WaveConfig()
+    _check(lib().pt_context_get_wave_config(ctx.handle, C.byref(cfg)))
+    return {"lds_stack_depth": cfg.lds_stack_depth, "chunk_paths": cfg.chunk_paths, "sort_key": cfg.sort_key}
+
+
+def _tracer_array(tracers, samples):
+    n = len(tracers)
+    handles = (C.c_void_p * n)(*[t.handle for t in tracers])
+    s = (C.c_uint32 * n)(*([samples] * n if isinstance(samples, int) else samples))
+    return n, handles, s
+
+
+def trace_batch(tracers, samples):
+    """Trace samples (int or per-tracer list) on every tracer in one mixed wavefront (pt_trace_batch)."""
+    n, h, s = _tracer_array(tracers, samples)
+    _check(lib().pt_trace_batch(h, n, s))
+
+
+def autotune_batch(tracers, samples):
+    n, h, s = _tracer_array(tracers, samples)
+    r = TuneResult()
+    _check(lib().pt_autotune_batch(h, n, s, C.byref(r)))
+    return {"best": {"lds_stack_depth": r.best.lds_stack_depth, "chunk_paths": r.best.chunk_paths,
+                     "sort_key": r.best.sort_key}, "best_ms": r.best_ms, "initial_ms": r.initial_ms,
+            "trials": r.trials}
+
+
+class Server:
+    """Render server with continuous batching (pt_server_*)."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes=200 << 30, max_active_jobs=64, round_paths=0):
+        h = C.c_void_p()
+        cfg = ServerConfig(hbm_budget_bytes, max_active_jobs, round_paths)
+        _check(lib().pt_server_create(ctx.handle, C.byref(cfg), C.byref(h)))
+        self.handle = h
+        self.ctx = ctx
+
+    def submit(self, scene: Scene, width, height, samples, max_bounces):
+        req = RenderRequest(scene.handle, width, height, samples, max_bounces)
+        jid = C.c_uint64()
+        _check(lib().pt_server_submit(self.handle, C.byref(req), C.byref(jid)))
+        return jid.value
+
+    def wait(self, job_id, timeout_ms=60_000):
+        _check(lib().pt_server_wait(self.handle, job_id, timeout_ms))
+
+    def info(self, job_id):
+        i = JobInfo()
+        _check(lib().pt_server_info(self.handle, job_id, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in JobInfo._fields_}
+
+    def result(self, job_id, width, height):
+        out = np.empty((height, width, 4), np.float32)
+        _check(lib().pt_server_result(self.handle, job_id, _ptr(out)))
+        return out
+
+    def cancel(self, job_id):
+        _check(lib().pt_server_cancel(self.handle, job_id))
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().pt_server_stats_get(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().pt_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

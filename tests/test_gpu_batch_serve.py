@@ -1,0 +1,202 @@
+"""GPU: batching of many scenes, the autotuner knobs, and the render server.
+
+Every per-pixel result must be bit-identical whether a job is traced alone,
+in a batch with other scenes, under any wave configuration (queue sorting,
+LDS stack depth, chunk size), or through the server mixed with other jobs:
+the path computation is deterministic and each pixel's samples are always
+accumulated in sample order.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+import binding as oracle
+import ptcore
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def scenes(ctx):
+    worlds = [ptcore.World.generate(ptcore.SCENE_CORNELL, 1, 256, 256),
+              ptcore.World.generate(ptcore.SCENE_SIZED, 4, 320, 240, 20000),
+              ptcore.World.generate(ptcore.SCENE_SIZED, 5, 200, 200, 120000)]
+    sc = [ptcore.Scene(ctx, w.desc) for w in worlds]
+    yield worlds, sc
+    for s in sc:
+        s.close()
+
+
+JOBS = [  # (scene index, width, height, spp, bounces)
+    (0, 96, 80, 4, 8), (1, 128, 96, 2, 6), (2, 64, 64, 16, 6), (1, 72, 40, 3, 3), (0, 33, 17, 5, 2),
+]
+
+
+def render_alone(ctx, sc, jobs):
+    out = []
+    for si, w, h, spp, mb in jobs:
+        t = ptcore.Tracer(ctx, sc[si], w, h)
+        t.set_max_bounces(mb)
+        t.trace_all(spp)
+        out.append(t.accumulation())
+        t.close()
+    return out
+
+
+def test_batch_equals_individual_renders(ctx, scenes):
+    _, sc = scenes
+    alone = render_alone(ctx, sc, JOBS)
+    tracers = []
+    for si, w, h, spp, mb in JOBS:
+        t = ptcore.Tracer(ctx, sc[si], w, h)
+        t.set_max_bounces(mb)
+        tracers.append(t)
+    ptcore.trace_batch(tracers, [j[3] for j in JOBS])
+    for t, ref, job in zip(tracers, alone, JOBS):
+        assert t.sample_count == job[3]
+        assert np.array_equal(t.accumulation(), ref), job
+        t.close()
+
+
+@pytest.mark.parametrize("cfg", [dict(sort_key=ptcore.SORT_OCTANT), dict(sort_key=ptcore.SORT_MATERIAL),
+                                 dict(lds_stack_depth=24), dict(lds_stack_depth=32), dict(chunk_paths=1 << 16)])
+def test_wave_configs_are_bit_identical(ctx, scenes, cfg):
+    _, sc = scenes
+    base = render_alone(ctx, sc, JOBS[:3])
+    try:
+        ptcore.set_wave_config(ctx, **{**dict(lds_stack_depth=16, chunk_paths=1 << 22, sort_key=0), **cfg})
+        got = render_alone(ctx, sc, JOBS[:3])
+        tracers = []
+        for si, w, h, spp, mb in JOBS[:3]:
+            t = ptcore.Tracer(ctx, sc[si], w, h)
+            t.set_max_bounces(mb)
+            tracers.append(t)
+        ptcore.trace_batch(tracers, [j[3] for j in JOBS[:3]])
+        batch = [t.accumulation() for t in tracers]
+        for t in tracers:
+            t.close()
+    finally:
+        ptcore.set_wave_config(ctx)
+    for a, b, c in zip(base, got, batch):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+def test_batch_matches_oracle(ctx, scenes):
+    worlds, sc = scenes
+    si, w, h, spp, mb = JOBS[1]
+    t = ptcore.Tracer(ctx, sc[si], w, h)
+    t.set_max_bounces(mb)
+    other = ptcore.Tracer(ctx, sc[2], 64, 64)
+    ptcore.trace_batch([t, other], [spp, 4])
+    ref = oracle.render_f32(worlds[si].desc, w, h, mb, spp)[0]
+    g, r = t.accumulation()[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
+    assert np.abs(g - r).sum() / np.abs(r).sum() <= 1e-4
+    t.close()
+    other.close()
+
+
+def test_autotune_picks_a_valid_config(ctx, scenes):
+    _, sc = scenes
+    tracers = []
+    for si, w, h, spp, mb in JOBS[:3]:
+        t = ptcore.Tracer(ctx, sc[si], w, h)
+        t.set_max_bounces(mb)
+        tracers.append(t)
+    try:
+        res = ptcore.autotune_batch(tracers, [j[3] for j in JOBS[:3]])
+        assert res["trials"] >= 8
+        assert res["best"]["lds_stack_depth"] in (16, 24, 32)
+        assert res["best"]["sort_key"] in (0, 1, 2)
+        assert res["best_ms"] <= res["initial_ms"] * 1.0001
+        assert all(t.sample_count == 0 for t in tracers)
+        ptcore.trace_batch(tracers, [j[3] for j in JOBS[:3]])
+        got = [t.accumulation() for t in tracers]
+    finally:
+        ptcore.set_wave_config(ctx)
+        for t in tracers:
+            t.close()
+    for a, b in zip(got, render_alone(ctx, sc, JOBS[:3])):
+        assert np.array_equal(a, b)
+
+
+def test_server_result_same_alone_and_in_the_mix(ctx, scenes):
+    _, sc = scenes
+    si, w, h, spp, mb = JOBS[2]
+    srv = ptcore.Server(ctx, hbm_budget_bytes=8 << 30, max_active_jobs=8, round_paths=1 << 15)
+    try:
+        a = srv.submit(sc[si], w, h, spp, mb)
+        srv.wait(a)
+        alone = srv.result(a, w, h)
+        # the same request among others submitted concurrently from threads
+        ids = {}
+
+        def submit(key, job):
+            ids[key] = srv.submit(sc[job[0]], job[1], job[2], job[3], job[4])
+
+        threads = [threading.Thread(target=submit, args=(k, j)) for k, j in enumerate(JOBS)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        for k, jid in ids.items():
+            srv.wait(jid)
+        infos = {k: srv.info(jid) for k, jid in ids.items()}
+        mixed = {k: srv.result(jid, JOBS[k][1], JOBS[k][2]) for k, jid in ids.items()}
+        stats = srv.stats()
+    finally:
+        srv.close()
+    assert np.array_equal(mixed[2], alone)
+    ref = render_alone(ctx, sc, JOBS)
+    for k in ids:
+        assert np.array_equal(mixed[k], ref[k]), JOBS[k]
+        assert infos[k]["state"] == ptcore.JOB_DONE and infos[k]["done_s"] >= infos[k]["submit_s"]
+    assert stats["jobs_done"] == len(JOBS) + 1
+    assert max(i["rounds"] for i in infos.values()) >= 1
+
+
+def test_server_rejects_over_budget_and_cancels(ctx, scenes):
+    _, sc = scenes
+    srv = ptcore.Server(ctx, hbm_budget_bytes=sc[0].footprint() + 64 * 64 * 16 + (1 << 10), max_active_jobs=1)
+    try:
+        with pytest.raises(ptcore.PtError) as e:
+            srv.submit(sc[0], 1024, 1024, 1, 1)
+        assert e.value.code == ptcore.PT_ERR_MEMORY
+        assert srv.stats()["jobs_rejected"] == 1
+        with pytest.raises(ptcore.PtError) as e:
+            srv.submit(sc[0], 0, 16, 1, 1)
+        assert e.value.code == ptcore.PT_ERR_INVALID
+    finally:
+        srv.close()
+    srv = ptcore.Server(ctx, hbm_budget_bytes=1 << 34, max_active_jobs=1, round_paths=1 << 12)
+    try:
+        first = srv.submit(sc[2], 200, 200, 16, 6)
+        queued = [srv.submit(sc[0], 64, 64, 2, 2) for _ in range(3)]
+        srv.cancel(queued[1])
+        with pytest.raises(ptcore.PtError) as e:
+            srv.wait(queued[1], 1000)
+        assert e.value.code == ptcore.PT_ERR_CANCELLED
+        srv.wait(first)
+        srv.wait(queued[0])
+        srv.wait(queued[2])
+        assert srv.info(queued[1])["state"] == ptcore.JOB_CANCELLED
+        with pytest.raises(ptcore.PtError) as e:
+            srv.result(queued[1], 64, 64)
+        assert e.value.code == ptcore.PT_ERR_STATE
+    finally:
+        srv.close()
+
+
+def test_server_wait_times_out(ctx, scenes):
+    _, sc = scenes
+    srv = ptcore.Server(ctx, hbm_budget_bytes=1 << 34, max_active_jobs=1, round_paths=1 << 12)
+    try:
+        big = srv.submit(sc[2], 256, 256, 16, 6)
+        waiting = srv.submit(sc[0], 32, 32, 1, 1)
+        with pytest.raises(ptcore.PtError) as e:
+            srv.wait(waiting, 0)
+        assert e.value.code == ptcore.PT_ERR_TIMEOUT
+        srv.wait(big)
+        srv.wait(waiting)
+    finally:
+        srv.close()
