@@ -121,7 +121,7 @@ pt_status pt_autotune_batch(pt_tracer* const* tracers, uint32_t count, const uin
     out->initial_ms = best_ms;
     // coordinate descent: one knob at a time, keep what is faster
     const uint32_t lds_opts[] = {16, 24, 32};
-    const uint32_t chunk_opts[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
+    const uint32_t chunk_opts[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
     const uint32_t sort_opts[] = {ptd::SORT_NONE, ptd::SORT_OCTANT, ptd::SORT_MATERIAL};
     for (int knob = 0; knob < 3; ++knob) {
         const uint32_t* opts = knob == 0 ? chunk_opts : (knob == 1 ? sort_opts : lds_opts);

@@ -28,7 +28,7 @@ enum SortKey : uint32_t { SORT_NONE = 0, SORT_OCTANT = 1, SORT_MATERIAL = 2 };
 // wavefront tuning (the autotuner sweeps these)
 struct WaveConfig {
     uint32_t lds_depth = 16;          // traversal stack entries in LDS (16/24/32)
-    uint32_t chunk_paths = 1u << 22;  // paths in flight per chunk (queue tile size)
+    uint32_t chunk_paths = 1u << 24;  // paths in flight per chunk (queue tile size)
     uint32_t sort_key = SORT_NONE;    // ray-queue ordering
 };
 

@@ -34,19 +34,48 @@ __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(0xffffffffu, __builtin_amdgcn_mbcnt_lo(0xffffffffu, 0u));
 }
 
-// Append for every lane with `pred` set: one atomic per wave, slots by prefix
-// count of the ballot. Must be reached by all active lanes of the wave.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+// Block-level staging of queue appends: lanes append to an LDS buffer
+// (wave-aggregated LDS atomic); the block moves the buffer to the global queue
+// with one global atomic when it could overflow on the next iteration and at
+// the end. This keeps contention on the global counter to a few thousand
+// atomics per launch instead of one per wave. All calls are block-uniform.
+constexpr uint32_t BQ_CAP = 2048;
+struct BlockQueue {
+    uint32_t* buf;   // LDS [BQ_CAP]
+    uint32_t* n;     // LDS count
+    uint32_t* base;  // LDS broadcast of the global slot
+};
+__device__ __forceinline__ void bq_init(BlockQueue& q) {
+    if (threadIdx.x == 0) *q.n = 0u;
+    __syncthreads();
+}
+__device__ __forceinline__ void bq_push(BlockQueue& q, bool pred, uint32_t v) {
     const uint64_t mask = __ballot(pred);
     const uint32_t prefix = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
     const uint32_t total = (uint32_t)__popcll(mask);
     uint32_t base = 0;
     if (total) {
         const int leader = __ffsll((unsigned long long)mask) - 1;
-        if ((int)lane_id() == leader) base = atomicAdd(counter, total);
+        if ((int)lane_id() == leader) base = atomicAdd(q.n, total);
         base = __shfl(base, leader);
     }
-    return base + prefix;
+    if (pred) q.buf[base + prefix] = v;
+}
+// move the staged entries to queue[*count ...] if `force` or the buffer
+// could overflow with one more block-sized iteration
+__device__ __forceinline__ void bq_flush(BlockQueue& q, uint32_t* __restrict__ count, uint32_t* __restrict__ queue,
+                                         bool force) {
+    __syncthreads();
+    const uint32_t cnt = *q.n;
+    if (cnt != 0u && (force || cnt > BQ_CAP - WAVE_BLOCK)) {
+        if (threadIdx.x == 0) *q.base = atomicAdd(count, cnt);
+        __syncthreads();
+        const uint32_t b = *q.base;
+        for (uint32_t i = threadIdx.x; i < cnt; i += WAVE_BLOCK) queue[b + i] = q.buf[i];
+        __syncthreads();
+        if (threadIdx.x == 0) *q.n = 0u;
+        __syncthreads();
+    }
 }
 
 // block order: 8x8 pixel squares, row-major over the tile
@@ -96,6 +125,10 @@ template <bool MULTI>
 __global__ __launch_bounds__(WAVE_BLOCK) void k_generate(DevScene s0, WaveJob j0, const DevScene* __restrict__ scenes,
                                                          const WaveJob* __restrict__ jobs, ChunkUnits units, PathState ps,
                                                          uint32_t* __restrict__ queue, uint32_t* __restrict__ counts) {
+    __shared__ uint32_t q_buf[BQ_CAP];
+    __shared__ uint32_t q_n, q_base;
+    BlockQueue bq{q_buf, &q_n, &q_base};
+    bq_init(bq);
     const uint32_t n = units.n_paths;
     const uint32_t stride = gridDim.x * WAVE_BLOCK;
     for (uint32_t base = blockIdx.x * WAVE_BLOCK; base < n; base += stride) {
@@ -126,17 +159,26 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_generate(DevScene s0, WaveJob j0
                 ps.job[i] = ji;
             }
         }
-        const uint32_t slot = wave_append(&counts[0], active);
-        if (active) queue[slot] = i;
+        bq_push(bq, active, i);
+        bq_flush(bq, &counts[0], queue, false);
     }
+    bq_flush(bq, &counts[0], queue, true);
 }
 
+// Traversal of one bounce: entries [0, n_closest) of the launch are the
+// closest-hit rays of `closest_queue`, entries [n_closest, n_closest +
+// n_shadow) the shadow rays of the previous bounce's `shadow_queue`. Merging
+// the two keeps the launches large; a shadow ray's NEE term is added to the
+// path's radiance here, after that bounce's emission (shade b) and before the
+// next emission (shade b + 1), which is the shader's order.
 template <uint32_t LDS_DEPTH, bool COUNT, bool MULTI>
-__global__ __launch_bounds__(WAVE_BLOCK) void k_extend(DevScene s0, const DevScene* __restrict__ scenes,
-                                                       const WaveJob* __restrict__ jobs, PathState ps,
-                                                       const uint32_t* __restrict__ queue,
-                                                       const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ ovf,
-                                                       unsigned long long* __restrict__ counters) {
+__global__ __launch_bounds__(WAVE_BLOCK) void k_trace(DevScene s0, const DevScene* __restrict__ scenes,
+                                                      const WaveJob* __restrict__ jobs, PathState ps,
+                                                      const uint32_t* __restrict__ closest_queue,
+                                                      const uint32_t* __restrict__ closest_count,
+                                                      const uint32_t* __restrict__ shadow_queue,
+                                                      const uint32_t* __restrict__ shadow_count,
+                                                      uint32_t* __restrict__ ovf, unsigned long long* __restrict__ counters) {
     __shared__ uint32_t lds_stack[LDS_DEPTH * WAVE_BLOCK];
     Stack st;
     st.lds = lds_stack + threadIdx.x;
@@ -145,58 +187,40 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_extend(DevScene s0, const DevSce
     st.ovf = ovf;
     st.ovf_stride = gridDim.x * WAVE_BLOCK;
     st.gtid = blockIdx.x * WAVE_BLOCK + threadIdx.x;
-    const uint32_t count = *count_ptr;
-    TraversalCounters ctr = {0, 0};
-    for (uint32_t j = blockIdx.x * WAVE_BLOCK + threadIdx.x; j < count; j += gridDim.x * WAVE_BLOCK) {
-        const uint32_t p = queue[j];
-        const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
-        const f3 o = ld3(ps.ox, ps.oy, ps.oz, p), d = ld3(ps.dx, ps.dy, ps.dz, p);
-        const f3 rcp = f3{1.0f, 1.0f, 1.0f} / d;
-        const bool opaque = s.all_opaque != 0u;
-        Rng rng{opaque ? 0u : ps.rng[p]};
-        float t, v, w;
-        const int32_t h = closest_hit<COUNT>(s, o, d, rcp, rng, st, t, v, w, ctr);
-        ps.hit[p] = h;
-        ps.ht[p] = t;
-        ps.hv[p] = v;
-        ps.hw[p] = w;
-        if (!opaque) ps.rng[p] = rng.s;
-    }
-    if (COUNT) {
-        atomicAdd(&counters[2], (unsigned long long)ctr.pairs);
-        atomicAdd(&counters[3], (unsigned long long)ctr.tris);
-    }
-}
-
-template <uint32_t LDS_DEPTH, bool COUNT, bool MULTI>
-__global__ __launch_bounds__(WAVE_BLOCK) void k_shadow(DevScene s0, const DevScene* __restrict__ scenes,
-                                                       const WaveJob* __restrict__ jobs, PathState ps,
-                                                       const uint32_t* __restrict__ queue,
-                                                       const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ ovf,
-                                                       unsigned long long* __restrict__ counters) {
-    __shared__ uint32_t lds_stack[LDS_DEPTH * WAVE_BLOCK];
-    Stack st;
-    st.lds = lds_stack + threadIdx.x;
-    st.lds_stride = WAVE_BLOCK;
-    st.lds_depth = LDS_DEPTH;
-    st.ovf = ovf;
-    st.ovf_stride = gridDim.x * WAVE_BLOCK;
-    st.gtid = blockIdx.x * WAVE_BLOCK + threadIdx.x;
-    const uint32_t count = *count_ptr;
-    TraversalCounters ctr = {0, 0};
-    for (uint32_t j = blockIdx.x * WAVE_BLOCK + threadIdx.x; j < count; j += gridDim.x * WAVE_BLOCK) {
-        const uint32_t p = queue[j];
-        const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
-        const f3 o = ld3(ps.sox, ps.soy, ps.soz, p), d = ld3(ps.sdx, ps.sdy, ps.sdz, p);
-        Rng rng{0u};  // only opaque scenes queue shadow rays: any-hit draws nothing
-        if (!occluded<COUNT>(s, o, d, ps.sdist[p], ps.slight[p], rng, st, ctr)) {
-            f3 L = ld3(ps.lr, ps.lg, ps.lb, p) + ld3(ps.scr, ps.scg, ps.scb, p);
-            st3(ps.lr, ps.lg, ps.lb, p, L);
+    const uint32_t nc = closest_queue ? *closest_count : 0u;
+    const uint32_t ns = shadow_queue ? *shadow_count : 0u;
+    TraversalCounters cc = {0, 0}, sc = {0, 0};
+    for (uint32_t j = blockIdx.x * WAVE_BLOCK + threadIdx.x; j < nc + ns; j += gridDim.x * WAVE_BLOCK) {
+        if (j < nc) {
+            const uint32_t p = closest_queue[j];
+            const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
+            const f3 o = ld3(ps.ox, ps.oy, ps.oz, p), d = ld3(ps.dx, ps.dy, ps.dz, p);
+            const f3 rcp = f3{1.0f, 1.0f, 1.0f} / d;
+            const bool opaque = s.all_opaque != 0u;
+            Rng rng{opaque ? 0u : ps.rng[p]};
+            float t, v, w;
+            const int32_t h = closest_hit<COUNT>(s, o, d, rcp, rng, st, t, v, w, cc);
+            ps.hit[p] = h;
+            ps.ht[p] = t;
+            ps.hv[p] = v;
+            ps.hw[p] = w;
+            if (!opaque) ps.rng[p] = rng.s;
+        } else {
+            const uint32_t p = shadow_queue[j - nc];
+            const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
+            const f3 o = ld3(ps.sox, ps.soy, ps.soz, p), d = ld3(ps.sdx, ps.sdy, ps.sdz, p);
+            Rng rng{0u};  // only opaque scenes queue shadow rays: any-hit draws nothing
+            if (!occluded<COUNT>(s, o, d, ps.sdist[p], ps.slight[p], rng, st, sc)) {
+                f3 L = ld3(ps.lr, ps.lg, ps.lb, p) + ld3(ps.scr, ps.scg, ps.scb, p);
+                st3(ps.lr, ps.lg, ps.lb, p, L);
+            }
         }
     }
     if (COUNT) {
-        atomicAdd(&counters[4], (unsigned long long)ctr.pairs);
-        atomicAdd(&counters[5], (unsigned long long)ctr.tris);
+        atomicAdd(&counters[2], (unsigned long long)cc.pairs);
+        atomicAdd(&counters[3], (unsigned long long)cc.tris);
+        atomicAdd(&counters[4], (unsigned long long)sc.pairs);
+        atomicAdd(&counters[5], (unsigned long long)sc.tris);
     }
 }
 
@@ -219,6 +243,11 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_shade(DevScene s0, const DevScen
     st.ovf = ovf;
     st.ovf_stride = gridDim.x * WAVE_BLOCK;
     st.gtid = blockIdx.x * WAVE_BLOCK + threadIdx.x;
+    __shared__ uint32_t nq_buf[BQ_CAP], sq_buf[BQ_CAP];
+    __shared__ uint32_t nq_n, nq_base, sq_n, sq_base;
+    BlockQueue nq{nq_buf, &nq_n, &nq_base}, sq{sq_buf, &sq_n, &sq_base};
+    bq_init(nq);
+    bq_init(sq);
     const uint32_t count = *count_ptr;
     const uint32_t stride = gridDim.x * WAVE_BLOCK;
     for (uint32_t base = blockIdx.x * WAVE_BLOCK; base < count; base += stride) {
@@ -298,11 +327,13 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_shade(DevScene s0, const DevScen
             }
             st3(ps.lr, ps.lg, ps.lb, p, L);
         }
-        const uint32_t ns = wave_append(next_count, cont);
-        if (cont) next_queue[ns] = p;
-        const uint32_t ss = wave_append(shadow_count, shadow);
-        if (shadow) shadow_queue[ss] = p;
+        bq_push(nq, cont, p);
+        bq_push(sq, shadow, p);
+        bq_flush(nq, next_count, next_queue, false);
+        bq_flush(sq, shadow_count, shadow_queue, false);
     }
+    bq_flush(nq, next_count, next_queue, true);
+    bq_flush(sq, shadow_count, shadow_queue, true);
 }
 
 // Per pixel of each unit, add the job's samples in order (trace.slang:1299-1308).
@@ -425,23 +456,13 @@ struct EventScope {
 };
 
 template <bool COUNT, bool MULTI>
-void launch_extend(uint32_t lds, uint32_t grid, hipStream_t st, const DevScene& s0, const DevScene* scenes,
-                   const WaveJob* jobs, const PathState& ps, const uint32_t* q, const uint32_t* cnt, uint32_t* ovf,
-                   unsigned long long* ctr) {
+void launch_trace(uint32_t lds, uint32_t grid, hipStream_t st, const DevScene& s0, const DevScene* scenes,
+                  const WaveJob* jobs, const PathState& ps, const uint32_t* cq, const uint32_t* cc, const uint32_t* sq,
+                  const uint32_t* scnt, uint32_t* ovf, unsigned long long* ctr) {
     switch (lds) {
-        case 24: k_extend<24, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, q, cnt, ovf, ctr); break;
-        case 32: k_extend<32, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, q, cnt, ovf, ctr); break;
-        default: k_extend<16, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, q, cnt, ovf, ctr); break;
-    }
-}
-template <bool COUNT, bool MULTI>
-void launch_shadow(uint32_t lds, uint32_t grid, hipStream_t st, const DevScene& s0, const DevScene* scenes,
-                   const WaveJob* jobs, const PathState& ps, const uint32_t* q, const uint32_t* cnt, uint32_t* ovf,
-                   unsigned long long* ctr) {
-    switch (lds) {
-        case 24: k_shadow<24, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, q, cnt, ovf, ctr); break;
-        case 32: k_shadow<32, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, q, cnt, ovf, ctr); break;
-        default: k_shadow<16, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, q, cnt, ovf, ctr); break;
+        case 24: k_trace<24, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, cq, cc, sq, scnt, ovf, ctr); break;
+        case 32: k_trace<32, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, cq, cc, sq, scnt, ovf, ctr); break;
+        default: k_trace<16, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, cq, cc, sq, scnt, ovf, ctr); break;
     }
 }
 
@@ -456,6 +477,19 @@ struct Launch {
 };
 
 template <bool MULTI>
+void sort_queue(Wavefront& w, const Launch& L, uint32_t key, const uint32_t* in, const uint32_t* cnt, uint32_t* out) {
+    const hipStream_t stream = L.stream;
+    k_sort_clear<<<1, SORT_BUCKETS, 0, stream>>>(w.sort_hist);
+    k_sort_hist<MULTI><<<L.grid, WAVE_BLOCK, 0, stream>>>(key, *L.s0, L.scenes, L.jobs, w.ps, in, cnt, w.sort_hist);
+    k_sort_scan<<<1, SORT_BUCKETS, 0, stream>>>(w.sort_hist);
+    k_sort_scatter<MULTI><<<L.grid, WAVE_BLOCK, 0, stream>>>(key, *L.s0, L.scenes, L.jobs, w.ps, in, cnt, w.sort_hist, out);
+}
+
+// One chunk: generate; per bounce b: trace (closest rays of b + shadow rays
+// of b - 1), shade b; a last shadow-only trace; accumulate. Ray queues rotate
+// among queue[0], queue[1], queue[3] (cur, next, sort scratch); shadow rays
+// use queue[2], consumed by the trace that follows the shade producing them.
+template <bool MULTI>
 void run_chunk(Wavefront& w, const Launch& L, uint32_t max_bounces, bool counting, EventLog* lg,
                unsigned long long* ctr) {
     const hipStream_t stream = L.stream;
@@ -467,48 +501,31 @@ void run_chunk(Wavefront& w, const Launch& L, uint32_t max_bounces, bool countin
     uint32_t* cur = w.queue[0];
     uint32_t* nxt = w.queue[1];
     uint32_t* spare = w.queue[3];
+    uint32_t* shadow = w.queue[2];
+    auto trace = [&](const uint32_t* cq, const uint32_t* cc, const uint32_t* sq, const uint32_t* sc) {
+        EventScope tm(lg, EV_EXTEND, stream);
+        if (counting) launch_trace<true, MULTI>(w.cfg.lds_depth, grid, stream, *L.s0, L.scenes, L.jobs, ps, cq, cc, sq, sc, w.ovf, ctr);
+        else launch_trace<false, MULTI>(w.cfg.lds_depth, grid, stream, *L.s0, L.scenes, L.jobs, ps, cq, cc, sq, sc, w.ovf, ctr);
+    };
     for (uint32_t b = 0; b <= max_bounces; ++b) {
         uint32_t* cnt = w.counts + 2 * b;
         if (w.cfg.sort_key == SORT_OCTANT && b > 0) {
-            k_sort_clear<<<1, SORT_BUCKETS, 0, stream>>>(w.sort_hist);
-            k_sort_hist<MULTI><<<grid, WAVE_BLOCK, 0, stream>>>(SORT_OCTANT, *L.s0, L.scenes, L.jobs, ps, cur, cnt, w.sort_hist);
-            k_sort_scan<<<1, SORT_BUCKETS, 0, stream>>>(w.sort_hist);
-            k_sort_scatter<MULTI><<<grid, WAVE_BLOCK, 0, stream>>>(SORT_OCTANT, *L.s0, L.scenes, L.jobs, ps, cur, cnt,
-                                                                    w.sort_hist, spare);
+            sort_queue<MULTI>(w, L, SORT_OCTANT, cur, cnt, spare);
             std::swap(cur, spare);
         }
-        {
-            EventScope tm(lg, EV_EXTEND, stream);
-            if (counting)
-                launch_extend<true, MULTI>(w.cfg.lds_depth, grid, stream, *L.s0, L.scenes, L.jobs, ps, cur, cnt, w.ovf, ctr);
-            else
-                launch_extend<false, MULTI>(w.cfg.lds_depth, grid, stream, *L.s0, L.scenes, L.jobs, ps, cur, cnt, w.ovf, ctr);
-        }
+        trace(cur, cnt, b > 0 ? shadow : nullptr, b > 0 ? cnt - 1 : nullptr);
         if (w.cfg.sort_key == SORT_MATERIAL) {
-            k_sort_clear<<<1, SORT_BUCKETS, 0, stream>>>(w.sort_hist);
-            k_sort_hist<MULTI><<<grid, WAVE_BLOCK, 0, stream>>>(SORT_MATERIAL, *L.s0, L.scenes, L.jobs, ps, cur, cnt, w.sort_hist);
-            k_sort_scan<<<1, SORT_BUCKETS, 0, stream>>>(w.sort_hist);
-            k_sort_scatter<MULTI><<<grid, WAVE_BLOCK, 0, stream>>>(SORT_MATERIAL, *L.s0, L.scenes, L.jobs, ps, cur, cnt,
-                                                                    w.sort_hist, spare);
+            sort_queue<MULTI>(w, L, SORT_MATERIAL, cur, cnt, spare);
             std::swap(cur, spare);
         }
         {
             EventScope tm(lg, EV_SHADE, stream);
             k_shade<MULTI><<<grid, WAVE_BLOCK, 0, stream>>>(*L.s0, L.scenes, L.jobs, *L.j0, ps, b, cur, cnt, nxt,
-                                                            w.counts + 2 * (b + 1), w.queue[2], cnt + 1, w.ovf);
-        }
-        {
-            EventScope tm(lg, EV_SHADOW, stream);
-            if (counting)
-                launch_shadow<true, MULTI>(w.cfg.lds_depth, grid, stream, *L.s0, L.scenes, L.jobs, ps, w.queue[2], cnt + 1, w.ovf, ctr);
-            else
-                launch_shadow<false, MULTI>(w.cfg.lds_depth, grid, stream, *L.s0, L.scenes, L.jobs, ps, w.queue[2], cnt + 1, w.ovf, ctr);
+                                                            w.counts + 2 * (b + 1), shadow, cnt + 1, w.ovf);
         }
         std::swap(cur, nxt);
-        // keep `spare` distinct from the live queues
-        if (spare == cur || spare == nxt) spare = (w.queue[0] != cur && w.queue[0] != nxt) ? w.queue[0]
-                                                 : (w.queue[1] != cur && w.queue[1] != nxt) ? w.queue[1] : w.queue[3];
     }
+    trace(nullptr, nullptr, shadow, w.counts + 2 * max_bounces + 1);
     const uint32_t acc_grid = std::min<uint32_t>(grid, (L.units.n_units * 64u + WAVE_BLOCK - 1) / WAVE_BLOCK);
     k_accumulate<MULTI><<<acc_grid, WAVE_BLOCK, 0, stream>>>(*L.j0, L.jobs, L.units, ps);
 }
