@@ -79,6 +79,10 @@ def main():
     ap.add_argument("--schedule", default="wavefront", choices=["wavefront", "mega"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=2)
+    ap.add_argument("--traversal", type=int, default=None, choices=[0, 1],
+                    help="wavefront traversal: 0 batch per wave, 1 lane refill (default: library default)")
+    ap.add_argument("--sort", type=int, default=None, choices=[0, 1, 2], help="queue sort key")
+    ap.add_argument("--lds", type=int, default=None, choices=[16, 24, 32], help="LDS stack depth")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -97,6 +101,10 @@ def main():
     ctx.synchronize()
     upload_s = time.perf_counter() - t0
     schedule = ptcore.SCHEDULE_WAVEFRONT if args.schedule == "wavefront" else ptcore.SCHEDULE_MEGAKERNEL
+    knobs = {k: v for k, v in (("traversal", args.traversal), ("sort_key", args.sort), ("lds_stack_depth", args.lds))
+             if v is not None}
+    if knobs:
+        ptcore.set_wave_config(ctx, **{**ptcore.get_wave_config(ctx), **knobs})
     tracer = ptcore.Tracer(ctx, scene, W, H, schedule)
     tracer.set_max_bounces(BOUNCES)
     print(f"[bench] {desc}; {world.desc.triangle_count} tris, host prep {prep_s:.2f}s, upload {upload_s:.2f}s",
@@ -150,21 +158,25 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": desc, "width": W, "height": H, "spp": SPP, "max_bounces": BOUNCES,
-                   "triangles": int(world.desc.triangle_count), "schedule": args.schedule},
+                   "triangles": int(world.desc.triangle_count), "schedule": args.schedule,
+                   "wave_config": ptcore.get_wave_config(ctx)},
         "rays_per_s": rays_per_step * args.steps / total_s,
         "p50_ms": percentile(lat, 50),
         "p99_ms": percentile(lat, 99),
     }
     if schedule == ptcore.SCHEDULE_WAVEFRONT:
+        # the traversal kernel walks the closest-hit rays of a bounce and the
+        # shadow rays of the previous one; its bytes are both traversals'
         extend_launches = max(ev["extend_launches"], 1)
-        bytes_per_step = PAIR_BYTES * counts["pair_tests"] + TRI_BYTES * counts["tri_tests"]
+        bytes_per_step = (PAIR_BYTES * (counts["pair_tests"] + counts["shadow_pair_tests"]) +
+                          TRI_BYTES * (counts["tri_tests"] + counts["shadow_tri_tests"]))
         launches_per_step = extend_launches / args.steps
         bytes_per_launch = bytes_per_step / launches_per_step
         avg_launch_s = ev["extend_ms"] / 1e3 / extend_launches
         achieved = bytes_per_launch / avg_launch_s / 1e9
         result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(args.workload),
-                              "kernel": "k_extend (closest-hit traversal)",
+                              "kernel": "k_trace (closest-hit rays of bounce b + shadow rays of bounce b-1)",
                               "avg_launch_ms": avg_launch_s * 1e3, "launches_per_step": launches_per_step,
                               "algorithmic_bytes_per_launch": bytes_per_launch}
         result["kernel_ms_per_step"] = {"extend": ev["extend_ms"] / args.steps, "shade": ev["shade_ms"] / args.steps,

@@ -229,6 +229,8 @@ typedef struct pt_wave_config {
     uint32_t lds_stack_depth;  /* traversal stack entries kept in LDS: 16, 24 or 32 */
     uint32_t chunk_paths;      /* paths in flight per chunk (the queue tile size) */
     uint32_t sort_key;         /* ray-queue order: 0 none, 1 direction octant, 2 hit material */
+    uint32_t traversal;        /* 0: each thread traces its rays one batch of the wave at a time;
+                                  1: lane refill (a lane that finishes starts its next ray at once) */
 } pt_wave_config;
 #define PT_SORT_NONE 0
 #define PT_SORT_OCTANT 1

@@ -35,6 +35,7 @@ pt_status check_config(const pt_wave_config* c) {
     if (c->chunk_paths < 4096 || c->chunk_paths > (1u << 27))
         return set_error(PT_ERR_INVALID, "chunk_paths must be in [4096, 2^27]");
     if (c->sort_key > PT_SORT_MATERIAL) return set_error(PT_ERR_INVALID, "unknown sort key %u", c->sort_key);
+    if (c->traversal > 1) return set_error(PT_ERR_INVALID, "unknown traversal mode %u", c->traversal);
     return PT_OK;
 }
 
@@ -80,6 +81,7 @@ pt_status pt_context_set_wave_config(pt_context* c, const pt_wave_config* cfg) {
     c->wave_cfg.lds_depth = cfg->lds_stack_depth;
     c->wave_cfg.chunk_paths = cfg->chunk_paths;
     c->wave_cfg.sort_key = cfg->sort_key;
+    c->wave_cfg.traversal = cfg->traversal;
     return PT_OK;
 }
 
@@ -88,6 +90,7 @@ pt_status pt_context_get_wave_config(const pt_context* c, pt_wave_config* cfg) {
     cfg->lds_stack_depth = c->wave_cfg.lds_depth;
     cfg->chunk_paths = c->wave_cfg.chunk_paths;
     cfg->sort_key = c->wave_cfg.sort_key;
+    cfg->traversal = c->wave_cfg.traversal;
     return PT_OK;
 }
 
@@ -123,14 +126,16 @@ pt_status pt_autotune_batch(pt_tracer* const* tracers, uint32_t count, const uin
     const uint32_t lds_opts[] = {16, 24, 32};
     const uint32_t chunk_opts[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
     const uint32_t sort_opts[] = {ptd::SORT_NONE, ptd::SORT_OCTANT, ptd::SORT_MATERIAL};
-    for (int knob = 0; knob < 3; ++knob) {
-        const uint32_t* opts = knob == 0 ? chunk_opts : (knob == 1 ? sort_opts : lds_opts);
-        const int nopts = knob == 0 ? 4 : 3;
+    const uint32_t traversal_opts[] = {0, 1};
+    for (int knob = 0; knob < 4; ++knob) {
+        const uint32_t* opts = knob == 0 ? traversal_opts : knob == 1 ? chunk_opts : (knob == 2 ? sort_opts : lds_opts);
+        const int nopts = knob == 0 ? 2 : (knob == 1 ? 4 : 3);
         ptd::WaveConfig base = best;
         for (int k = 0; k < nopts; ++k) {
             ptd::WaveConfig cand = base;
-            if (knob == 0) cand.chunk_paths = opts[k];
-            else if (knob == 1) cand.sort_key = opts[k];
+            if (knob == 0) cand.traversal = opts[k];
+            else if (knob == 1) cand.chunk_paths = opts[k];
+            else if (knob == 2) cand.sort_key = opts[k];
             else cand.lds_depth = opts[k];
             if (std::memcmp(&cand, &base, sizeof(cand)) == 0) continue;
             if ((st = trial(cand, ms))) return st;
@@ -146,6 +151,7 @@ pt_status pt_autotune_batch(pt_tracer* const* tracers, uint32_t count, const uin
     out->best.lds_stack_depth = best.lds_depth;
     out->best.chunk_paths = best.chunk_paths;
     out->best.sort_key = best.sort_key;
+    out->best.traversal = best.traversal;
     out->best_ms = best_ms;
     out->trials = trials;
     return PT_OK;

@@ -224,6 +224,84 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_trace(DevScene s0, const DevScen
     }
 }
 
+// The same work as k_trace with lane refill: each lane walks its own ray one
+// step per iteration and, when it finishes, stores the result and starts its
+// next ray (grid-stride order) in the same iteration, so a wave runs for about
+// the average ray cost of its lanes instead of its slowest ray per batch.
+template <uint32_t LDS_DEPTH, bool COUNT, bool MULTI>
+__global__ __launch_bounds__(WAVE_BLOCK) void k_trace_refill(DevScene s0, const DevScene* __restrict__ scenes,
+                                                             const WaveJob* __restrict__ jobs, PathState ps,
+                                                             const uint32_t* __restrict__ closest_queue,
+                                                             const uint32_t* __restrict__ closest_count,
+                                                             const uint32_t* __restrict__ shadow_queue,
+                                                             const uint32_t* __restrict__ shadow_count,
+                                                             uint32_t* __restrict__ ovf,
+                                                             unsigned long long* __restrict__ counters) {
+    __shared__ uint32_t lds_stack[LDS_DEPTH * WAVE_BLOCK];
+    Stack st;
+    st.lds = lds_stack + threadIdx.x;
+    st.lds_stride = WAVE_BLOCK;
+    st.lds_depth = LDS_DEPTH;
+    st.ovf = ovf;
+    st.ovf_stride = gridDim.x * WAVE_BLOCK;
+    st.gtid = blockIdx.x * WAVE_BLOCK + threadIdx.x;
+    const uint32_t nc = closest_queue ? *closest_count : 0u;
+    const uint32_t ns = shadow_queue ? *shadow_count : 0u;
+    const uint32_t total = nc + ns;
+    const uint32_t stride = gridDim.x * WAVE_BLOCK;
+    TraversalCounters cc = {0, 0}, sc = {0, 0};
+    uint32_t j = blockIdx.x * WAVE_BLOCK + threadIdx.x;
+    RayState r;
+    uint32_t p = 0;
+    Rng rng{0u};
+    bool opaque = true;
+    auto begin = [&](uint32_t jj) {
+        if (jj < nc) {
+            p = closest_queue[jj];
+            const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
+            opaque = s.all_opaque != 0u;
+            rng.s = opaque ? 0u : ps.rng[p];
+            ray_begin(s, r, ld3(ps.ox, ps.oy, ps.oz, p), ld3(ps.dx, ps.dy, ps.dz, p), false, 0.0f, -1, st);
+        } else {
+            p = shadow_queue[jj - nc];
+            const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
+            opaque = true;  // only opaque scenes queue shadow rays
+            rng.s = 0u;
+            ray_begin(s, r, ld3(ps.sox, ps.soy, ps.soz, p), ld3(ps.sdx, ps.sdy, ps.sdz, p), true, ps.sdist[p],
+                      ps.slight[p], st);
+        }
+    };
+    bool have = j < total;
+    if (have) begin(j);
+    while (have) {
+        const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
+        const bool done = r.shadow ? ray_step<COUNT>(s, r, rng, st, sc) : ray_step<COUNT>(s, r, rng, st, cc);
+        if (done) {
+            if (r.shadow) {
+                if (r.hit == 0) {
+                    f3 L = ld3(ps.lr, ps.lg, ps.lb, p) + ld3(ps.scr, ps.scg, ps.scb, p);
+                    st3(ps.lr, ps.lg, ps.lb, p, L);
+                }
+            } else {
+                ps.hit[p] = r.hit;
+                ps.ht[p] = r.hit_t;
+                ps.hv[p] = r.hit_v;
+                ps.hw[p] = r.hit_w;
+                if (!opaque) ps.rng[p] = rng.s;
+            }
+            j += stride;
+            have = j < total;
+            if (have) begin(j);
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&counters[2], (unsigned long long)cc.pairs);
+        atomicAdd(&counters[3], (unsigned long long)cc.tris);
+        atomicAdd(&counters[4], (unsigned long long)sc.pairs);
+        atomicAdd(&counters[5], (unsigned long long)sc.tris);
+    }
+}
+
 // One bounce of shading for the paths in `queue` (trace.slang:886-1296).
 // Paths of scenes with alpha-tested materials test their shadow ray inline
 // (the any-hit draws random numbers between NEE and BSDF sampling).
@@ -456,14 +534,24 @@ struct EventScope {
 };
 
 template <bool COUNT, bool MULTI>
-void launch_trace(uint32_t lds, uint32_t grid, hipStream_t st, const DevScene& s0, const DevScene* scenes,
-                  const WaveJob* jobs, const PathState& ps, const uint32_t* cq, const uint32_t* cc, const uint32_t* sq,
-                  const uint32_t* scnt, uint32_t* ovf, unsigned long long* ctr) {
-    switch (lds) {
-        case 24: k_trace<24, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, cq, cc, sq, scnt, ovf, ctr); break;
-        case 32: k_trace<32, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, cq, cc, sq, scnt, ovf, ctr); break;
-        default: k_trace<16, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, cq, cc, sq, scnt, ovf, ctr); break;
+void launch_trace(uint32_t lds, uint32_t refill, uint32_t grid, hipStream_t st, const DevScene& s0,
+                  const DevScene* scenes, const WaveJob* jobs, const PathState& ps, const uint32_t* cq,
+                  const uint32_t* cc, const uint32_t* sq, const uint32_t* scnt, uint32_t* ovf, unsigned long long* ctr) {
+#define PT_TRACE_LAUNCH(K, D) K<D, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, cq, cc, sq, scnt, ovf, ctr)
+    if (refill) {
+        switch (lds) {
+            case 24: PT_TRACE_LAUNCH(k_trace_refill, 24); break;
+            case 32: PT_TRACE_LAUNCH(k_trace_refill, 32); break;
+            default: PT_TRACE_LAUNCH(k_trace_refill, 16); break;
+        }
+    } else {
+        switch (lds) {
+            case 24: PT_TRACE_LAUNCH(k_trace, 24); break;
+            case 32: PT_TRACE_LAUNCH(k_trace, 32); break;
+            default: PT_TRACE_LAUNCH(k_trace, 16); break;
+        }
     }
+#undef PT_TRACE_LAUNCH
 }
 
 struct Launch {
@@ -504,8 +592,12 @@ void run_chunk(Wavefront& w, const Launch& L, uint32_t max_bounces, bool countin
     uint32_t* shadow = w.queue[2];
     auto trace = [&](const uint32_t* cq, const uint32_t* cc, const uint32_t* sq, const uint32_t* sc) {
         EventScope tm(lg, EV_EXTEND, stream);
-        if (counting) launch_trace<true, MULTI>(w.cfg.lds_depth, grid, stream, *L.s0, L.scenes, L.jobs, ps, cq, cc, sq, sc, w.ovf, ctr);
-        else launch_trace<false, MULTI>(w.cfg.lds_depth, grid, stream, *L.s0, L.scenes, L.jobs, ps, cq, cc, sq, sc, w.ovf, ctr);
+        if (counting)
+            launch_trace<true, MULTI>(w.cfg.lds_depth, w.cfg.traversal, grid, stream, *L.s0, L.scenes, L.jobs, ps, cq, cc, sq,
+                                      sc, w.ovf, ctr);
+        else
+            launch_trace<false, MULTI>(w.cfg.lds_depth, w.cfg.traversal, grid, stream, *L.s0, L.scenes, L.jobs, ps, cq, cc,
+                                       sq, sc, w.ovf, ctr);
     };
     for (uint32_t b = 0; b <= max_bounces; ++b) {
         uint32_t* cnt = w.counts + 2 * b;

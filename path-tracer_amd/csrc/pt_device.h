@@ -469,6 +469,97 @@ __device__ __forceinline__ bool occluded(const DevScene& s, f3 o, f3 d, float hi
     return false;
 }
 
+// ------------------------------------------------------------------ step-wise traversal
+// The same closest-hit / any-hit walks as above, cut into single steps so a
+// lane can finish its ray and start the next one while the other lanes of
+// its wave are still traversing (no wave-wide wait for the slowest ray).
+// Visiting order, tests and acceptance are exactly those of closest_hit and
+// occluded, so results are identical.
+struct RayState {
+    f3 o, rcp, neg;
+    Watertight wt;
+    float hit_t, hit_v, hit_w;
+    int32_t hit;       // closest: triangle index; shadow: 1 once occluded
+    int32_t light;     // shadow: light triangle to skip
+    uint32_t cur;      // current node code
+    bool shadow;
+};
+
+__device__ __forceinline__ void ray_begin(const DevScene& s, RayState& r, f3 o, f3 d, bool shadow, float tmax,
+                                          int32_t light, Stack& st) {
+    r.o = o;
+    r.rcp = f3{1.0f, 1.0f, 1.0f} / d;
+    r.neg = (-o) * r.rcp;
+    r.wt = watertight_setup(d, r.rcp);
+    r.hit_t = shadow ? tmax : ptm::FLT_MAX_;
+    r.hit_v = r.hit_w = 0.0f;
+    r.hit = shadow ? 0 : -1;
+    r.light = light;
+    r.shadow = shadow;
+    PtBvhNode root = s.bvh[0];
+    r.cur = child_code(0u, root.left_first, root.tri_count);
+    st.sp = 0;
+}
+
+// One traversal step; returns true when the ray is finished.
+template <bool COUNT>
+__device__ __forceinline__ bool ray_step(const DevScene& s, RayState& r, Rng& rng, Stack& st, TraversalCounters& ctr) {
+    const float4* __restrict__ nodes = reinterpret_cast<const float4*>(s.bvh);
+    if ((r.cur & (CODE_SMALL_LEAF | CODE_BIG_LEAF)) == 0u) {
+        const uint32_t cur = r.cur;
+        float4 l0 = nodes[2u * cur], l1 = nodes[2u * cur + 1u];
+        float4 r0 = nodes[2u * cur + 2u], r1 = nodes[2u * cur + 3u];
+        if (COUNT) ++ctr.pairs;
+        float dl = intersect_aabb(l0, l1, r.neg, r.rcp, r.hit_t);
+        float dr = intersect_aabb(r0, r1, r.neg, r.rcp, r.hit_t);
+        bool hl = dl >= 0.0f, hr = dr >= 0.0f;
+        uint32_t cl = child_code(cur, __float_as_uint(l0.w), __float_as_uint(l1.w));
+        uint32_t cr = child_code(cur + 1u, __float_as_uint(r0.w), __float_as_uint(r1.w));
+        if (hl && hr) {
+            bool left_closer = dl < dr;
+            st.push(left_closer ? cr : cl);
+            r.cur = left_closer ? cl : cr;
+            return false;
+        }
+        if (hl || hr) {
+            r.cur = hl ? cl : cr;
+            return false;
+        }
+    } else {
+        uint32_t first, count;
+        if (r.cur & CODE_SMALL_LEAF) {
+            first = (r.cur >> 1) & 0x3fffffffu;
+            count = (r.cur & 1u) + 1u;
+        } else {
+            PtBvhNode n = s.bvh[r.cur & 0x3fffffffu];
+            first = n.left_first;
+            count = n.tri_count;
+        }
+        for (uint32_t i = 0; i < count; ++i) {
+            const uint32_t ti = first + i;
+            if (r.shadow && (int32_t)ti == r.light) continue;
+            if (COUNT) ++ctr.tris;
+            float tt, bv, bw;
+            if (!tri_test(s.geo[ti].v, r.o, r.wt, r.hit_t, tt, bv, bw)) continue;
+            if (r.shadow) {
+                if (!passes_through(s, ti, bv, bw, rng)) {
+                    r.hit = 1;
+                    return true;
+                }
+            } else {
+                if (passes_through(s, ti, bv, bw, rng)) continue;
+                r.hit_t = tt;
+                r.hit = (int32_t)ti;
+                r.hit_v = bv;
+                r.hit_w = bw;
+            }
+        }
+    }
+    if (st.sp == 0u) return true;
+    r.cur = st.pop();
+    return false;
+}
+
 // ------------------------------------------------------------------ camera ray (trace.slang:736-748)
 __device__ __forceinline__ f3 primary_direction(const DevScene& s, uint32_t px, uint32_t py, uint32_t iw, uint32_t ih,
                                                 Rng& rng) {

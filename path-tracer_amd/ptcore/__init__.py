@@ -128,7 +128,8 @@ class TraceStats(C.Structure):
 
 
 class WaveConfig(C.Structure):
-    _fields_ = [("lds_stack_depth", C.c_uint32), ("chunk_paths", C.c_uint32), ("sort_key", C.c_uint32)]
+    _fields_ = [("lds_stack_depth", C.c_uint32), ("chunk_paths", C.c_uint32), ("sort_key", C.c_uint32),
+                ("traversal", C.c_uint32)]
 
 
 class TuneResult(C.Structure):
@@ -610,15 +611,20 @@ class Tracer:
             pass
 
 
-def set_wave_config(ctx: Context, lds_stack_depth=16, chunk_paths=1 << 22, sort_key=SORT_NONE):
-    cfg = WaveConfig(lds_stack_depth, chunk_paths, sort_key)
+DEFAULT_WAVE_CONFIG = dict(lds_stack_depth=16, chunk_paths=1 << 24, sort_key=SORT_NONE, traversal=0)
+
+
+def set_wave_config(ctx: Context, **overrides):
+    """Set the context's wavefront tuning; unspecified knobs take the defaults."""
+    v = {**DEFAULT_WAVE_CONFIG, **overrides}
+    cfg = WaveConfig(v["lds_stack_depth"], v["chunk_paths"], v["sort_key"], v["traversal"])
     _check(lib().pt_context_set_wave_config(ctx.handle, C.byref(cfg)))
 
 
 def get_wave_config(ctx: Context):
     cfg = WaveConfig()
     _check(lib().pt_context_get_wave_config(ctx.handle, C.byref(cfg)))
-    return {"lds_stack_depth": cfg.lds_stack_depth, "chunk_paths": cfg.chunk_paths, "sort_key": cfg.sort_key}
+    return {k: getattr(cfg, k) for k, _ in WaveConfig._fields_}
 
 
 def _tracer_array(tracers, samples):
@@ -638,9 +644,8 @@ def autotune_batch(tracers, samples):
     n, h, s = _tracer_array(tracers, samples)
     r = TuneResult()
     _check(lib().pt_autotune_batch(h, n, s, C.byref(r)))
-    return {"best": {"lds_stack_depth": r.best.lds_stack_depth, "chunk_paths": r.best.chunk_paths,
-                     "sort_key": r.best.sort_key}, "best_ms": r.best_ms, "initial_ms": r.initial_ms,
-            "trials": r.trials}
+    return {"best": {k: getattr(r.best, k) for k, _ in WaveConfig._fields_}, "best_ms": r.best_ms,
+            "initial_ms": r.initial_ms, "trials": r.trials}
 
 
 class Server:

@@ -60,12 +60,13 @@ def test_batch_equals_individual_renders(ctx, scenes):
 
 
 @pytest.mark.parametrize("cfg", [dict(sort_key=ptcore.SORT_OCTANT), dict(sort_key=ptcore.SORT_MATERIAL),
-                                 dict(lds_stack_depth=24), dict(lds_stack_depth=32), dict(chunk_paths=1 << 16)])
+                                 dict(lds_stack_depth=24), dict(lds_stack_depth=32), dict(chunk_paths=1 << 16),
+                                 dict(traversal=1), dict(traversal=1, sort_key=ptcore.SORT_OCTANT, lds_stack_depth=24)])
 def test_wave_configs_are_bit_identical(ctx, scenes, cfg):
     _, sc = scenes
     base = render_alone(ctx, sc, JOBS[:3])
     try:
-        ptcore.set_wave_config(ctx, **{**dict(lds_stack_depth=16, chunk_paths=1 << 22, sort_key=0), **cfg})
+        ptcore.set_wave_config(ctx, **cfg)
         got = render_alone(ctx, sc, JOBS[:3])
         tracers = []
         for si, w, h, spp, mb in JOBS[:3]:
