@@ -70,12 +70,180 @@ def load_traffic(name):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def build_worlds(specs):
+    """Prepare many procedural worlds in parallel host threads (the C++ build releases the GIL)."""
+    import ptcore
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(spec):
+        kind, seed, w, h, tris = spec
+        return ptcore.World.generate(kind, seed, w, h, tris)
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 4)) as ex:
+        return list(ex.map(one, specs))
+
+
+def mixed_jobs(seed, count):
+    """The mixed-scene batch: triangle counts log-uniform over 1k-500k,
+    resolutions uniform over {256, 512, 768, 1024}^2, spp uniform over {4, 16, 64}, depth 6."""
+    import random
+
+    rng = random.Random(seed)
+    jobs = []
+    for i in range(count):
+        tris = int(math.exp(rng.uniform(math.log(1000), math.log(500000))))
+        res = rng.choice([256, 512, 768, 1024])
+        spp = rng.choice([4, 16, 64])
+        jobs.append(dict(tris=tris, width=res, height=res, spp=spp, bounces=6, seed=1000 + i))
+    return jobs
+
+
+def bench_batch(args):
+    """Mixed-scene batch: every job traced by the plain single-scene megakernel one after
+    another, then all jobs as one autotuned mixed wavefront batch."""
+    import ptcore
+
+    jobs = mixed_jobs(4, args.batch_scenes)
+    t0 = time.perf_counter()
+    worlds = build_worlds([(ptcore.SCENE_SIZED, j["seed"], j["width"], j["height"], j["tris"]) for j in jobs])
+    prep_s = time.perf_counter() - t0
+    ctx = ptcore.Context(0)
+    scenes = [ptcore.Scene(ctx, w.desc) for w in worlds]
+    samples = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+    print(f"[bench] batch of {len(jobs)} scenes, {sum(w.desc.triangle_count for w in worlds)} triangles, "
+          f"{samples / 1e6:.1f}M path samples, host prep {prep_s:.1f}s", file=sys.stderr, flush=True)
+
+    def tracers(schedule):
+        ts = []
+        for j, s in zip(jobs, scenes):
+            t = ptcore.Tracer(ctx, s, j["width"], j["height"], schedule)
+            t.set_max_bounces(j["bounces"])
+            ts.append(t)
+        return ts
+
+    # plain method: megakernel, one scene at a time
+    mk = tracers(ptcore.SCHEDULE_MEGAKERNEL)
+    for t, j in zip(mk[:2], jobs[:2]):  # warm the code objects
+        t.trace_all(1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for t, j in zip(mk, jobs):
+        t.set_sample_count(0)
+        t.trace_all(j["spp"])
+    ctx.synchronize()
+    mega_s = time.perf_counter() - t0
+    for t in mk:
+        t.close()
+    # wavefront, one scene at a time (no batching)
+    wf = tracers(ptcore.SCHEDULE_WAVEFRONT)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for t, j in zip(wf, jobs):
+        t.set_sample_count(0)
+        t.trace_all(j["spp"])
+    ctx.synchronize()
+    single_wf_s = time.perf_counter() - t0
+    # batched: autotune, then timed steps of the whole batch
+    spp = [j["spp"] for j in jobs]
+    tune = ptcore.autotune_batch(wf, spp) if not args.no_autotune else None
+    print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
+    times = []
+    for k in range(args.warmup + args.steps):
+        for t in wf:
+            t.set_sample_count(0)
+        ctx.synchronize()
+        s0 = time.perf_counter()
+        ptcore.trace_batch(wf, spp)
+        ctx.synchronize()
+        if k >= args.warmup:
+            times.append(time.perf_counter() - s0)
+    batch_s = sum(times) / len(times)
+    for t in wf:
+        t.close()
+    value = samples / batch_s
+    return {
+        "metric": "path samples/s @1080p depth 6; rays/s; p50/p99 render-job latency",
+        "value": value, "unit": "path samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": batch_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded procedural scenes)",
+        "config": {"workload": f"mixed batch: {len(jobs)} scenes, triangles log-uniform 1k-500k, resolutions "
+                               f"{{256..1024}}^2, spp {{4,16,64}}, depth 6",
+                   "path_samples_per_step": samples, "wave_config": ptcore.get_wave_config(ctx), "autotune": tune},
+        "megakernel_sequential": {"value": samples / mega_s, "seconds": mega_s},
+        "wavefront_sequential": {"value": samples / single_wf_s, "seconds": single_wf_s},
+        "batched_vs_megakernel": (samples / batch_s) / (samples / mega_s),
+    }
+
+
+def bench_serve(args):
+    """Serving: Poisson arrivals of render requests over resident scenes (one
+    4.2M-triangle scene and two 262k-triangle scenes); each request draws its
+    resolution from {256..1024}^2 and spp from {4, 16, 64}, depth 6. For each
+    arrival rate: p50/p99 submit-to-result latency and sustained samples/s."""
+    import random
+    import threading
+
+    import ptcore
+
+    t0 = time.perf_counter()
+    worlds = build_worlds([(ptcore.SCENE_LARGE, 3, 1024, 1024, 0), (ptcore.SCENE_FIELD, 2, 1024, 1024, 0),
+                           (ptcore.SCENE_FIELD, 12, 1024, 1024, 0)])
+    print(f"[bench] serving pool prepared in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    ctx = ptcore.Context(0)
+    scenes = [ptcore.Scene(ctx, w.desc) for w in worlds]
+    results = []
+    for lam in args.rates:
+        srv = ptcore.Server(ctx, hbm_budget_bytes=200 << 30, max_active_jobs=64)
+        rng = random.Random(int(lam * 1000) + 5)
+        reqs = [(rng.randrange(len(scenes)), rng.choice([256, 512, 768, 1024]), rng.choice([4, 16, 64]))
+                for _ in range(args.jobs_per_rate)]
+        gaps = [rng.expovariate(lam) for _ in reqs]
+        ids = []
+
+        def producer():
+            t_next = time.perf_counter()
+            for (si, res, spp), gap in zip(reqs, gaps):
+                t_next += gap
+                d = t_next - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids.append((srv.submit(scenes[si], res, res, spp, 6), res, spp))
+
+        th = threading.Thread(target=producer)
+        th.start()
+        th.join()
+        lat, first, last, samples = [], None, 0.0, 0
+        for jid, res, spp in ids:
+            srv.wait(jid, 600_000)
+            info = srv.info(jid)
+            srv.result(jid, res, res)
+            lat.append((info["done_s"] - info["submit_s"]) * 1e3)
+            first = info["submit_s"] if first is None else min(first, info["submit_s"])
+            last = max(last, info["done_s"])
+            samples += res * res * spp
+        st = srv.stats()
+        srv.close()
+        r = {"rate_jobs_per_s": lam, "jobs": len(ids), "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+             "samples_per_s": samples / (last - first), "rounds": st["rounds"], "busy_s": st["busy_s"]}
+        print(f"[bench] serve {r}", file=sys.stderr, flush=True)
+        results.append(r)
+    return {"metric": "p50/p99 render-job latency under Poisson arrivals; path samples/s",
+            "unit": "ms", "n_gpus": 1, "data": "synthetic (seeded procedural scenes)",
+            "config": {"workload": "serving: pool of one 4.2M-tri and two 262k-tri resident scenes; requests "
+                                   "{256..1024}^2, spp {4,16,64}, depth 6; 200 GB HBM budget, 64 slots"},
+            "sweep": results}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="field", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="field", choices=sorted(WORKLOADS) + ["batch", "serve", "copy"])
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--rates", type=float, nargs="+", default=[5, 10, 20, 50, 100, 200])
+    ap.add_argument("--jobs-per-rate", type=int, default=40)
     ap.add_argument("--schedule", default="wavefront", choices=["wavefront", "mega"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=2)
@@ -91,6 +259,22 @@ def main():
 
     import ptcore
 
+    if args.workload in ("batch", "serve"):
+        out = bench_batch(args) if args.workload == "batch" else bench_serve(args)
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload == "copy":
+        # calibration for the HBM traffic counters: a plain device copy of known size
+        import torch
+
+        n = 1 << 28  # 1 GiB of float32
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_dispatch": n * 4, "dispatches": 4}), flush=True)
+        return
     kind_name, W, H, SPP, BOUNCES, desc = WORKLOADS[args.workload]
     t0 = time.perf_counter()
     world = ptcore.World.generate(getattr(ptcore, kind_name), {"field": 2, "cornell": 1, "large": 3}[args.workload], W, H)

@@ -440,8 +440,13 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_accumulate(WaveJob j0, const Wav
 }
 
 // ------------------------------------------------------------------ queue sorting
-// Counting sort of a queue by a small key (direction octant with the scene in
-// the high bits, or hit material), so waves see coherent rays / materials.
+// Block-local counting sort of a queue by a small key (direction octant with
+// the scene in the high bits, or hit material): each block sorts contiguous
+// segments of SORT_SEG entries in LDS, so the lanes of a wave see coherent
+// rays / materials while each segment stays where it was in the queue (the
+// path state it points at keeps its locality).
+constexpr uint32_t SORT_SEG = 2048;
+
 template <bool MULTI>
 __device__ __forceinline__ uint32_t sort_key(uint32_t key_kind, const DevScene& s0, const DevScene* scenes,
                                              const WaveJob* jobs, const PathState& ps, uint32_t p) {
@@ -456,50 +461,40 @@ __device__ __forceinline__ uint32_t sort_key(uint32_t key_kind, const DevScene& 
     return min(s.attr[h].material, SORT_BUCKETS - 2u);
 }
 
-__global__ void k_sort_clear(uint32_t* hist) {
-    if (threadIdx.x < SORT_BUCKETS) hist[threadIdx.x] = 0u;
-}
-
 template <bool MULTI>
-__global__ __launch_bounds__(WAVE_BLOCK) void k_sort_hist(uint32_t key_kind, DevScene s0, const DevScene* __restrict__ scenes,
-                                                          const WaveJob* __restrict__ jobs, PathState ps,
-                                                          const uint32_t* __restrict__ queue,
-                                                          const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t local[SORT_BUCKETS];
-    local[threadIdx.x] = 0u;
-    __syncthreads();
+__global__ __launch_bounds__(WAVE_BLOCK) void k_sort_local(uint32_t key_kind, DevScene s0, const DevScene* __restrict__ scenes,
+                                                           const WaveJob* __restrict__ jobs, PathState ps,
+                                                           const uint32_t* __restrict__ queue,
+                                                           const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ out) {
+    __shared__ uint32_t vals[SORT_SEG];
+    __shared__ uint8_t keys[SORT_SEG];
+    __shared__ uint32_t hist[SORT_BUCKETS];
     const uint32_t count = *count_ptr;
-    for (uint32_t j = blockIdx.x * WAVE_BLOCK + threadIdx.x; j < count; j += gridDim.x * WAVE_BLOCK)
-        atomicAdd(&local[sort_key<MULTI>(key_kind, s0, scenes, jobs, ps, queue[j])], 1u);
-    __syncthreads();
-    if (local[threadIdx.x]) atomicAdd(&hist[threadIdx.x], local[threadIdx.x]);
-}
-
-// exclusive scan of the 256 bucket counts in place (one block)
-__global__ __launch_bounds__(SORT_BUCKETS) void k_sort_scan(uint32_t* hist) {
-    __shared__ uint32_t v[SORT_BUCKETS];
     const uint32_t t = threadIdx.x;
-    v[t] = hist[t];
-    __syncthreads();
-    for (uint32_t off = 1; off < SORT_BUCKETS; off <<= 1) {
-        uint32_t add = t >= off ? v[t - off] : 0u;
+    for (uint32_t seg = blockIdx.x * SORT_SEG; seg < count; seg += gridDim.x * SORT_SEG) {
+        const uint32_t n = min(SORT_SEG, count - seg);
+        hist[t] = 0u;
         __syncthreads();
-        v[t] += add;
+        for (uint32_t i = t; i < n; i += WAVE_BLOCK) {
+            const uint32_t p = queue[seg + i];
+            const uint32_t k = sort_key<MULTI>(key_kind, s0, scenes, jobs, ps, p);
+            vals[i] = p;
+            keys[i] = (uint8_t)k;
+            atomicAdd(&hist[k], 1u);
+        }
         __syncthreads();
-    }
-    hist[t] = v[t] - hist[t];
-}
-
-template <bool MULTI>
-__global__ __launch_bounds__(WAVE_BLOCK) void k_sort_scatter(uint32_t key_kind, DevScene s0, const DevScene* __restrict__ scenes,
-                                                             const WaveJob* __restrict__ jobs, PathState ps,
-                                                             const uint32_t* __restrict__ queue,
-                                                             const uint32_t* __restrict__ count_ptr, uint32_t* __restrict__ hist,
-                                                             uint32_t* __restrict__ out) {
-    const uint32_t count = *count_ptr;
-    for (uint32_t j = blockIdx.x * WAVE_BLOCK + threadIdx.x; j < count; j += gridDim.x * WAVE_BLOCK) {
-        const uint32_t p = queue[j];
-        out[atomicAdd(&hist[sort_key<MULTI>(key_kind, s0, scenes, jobs, ps, p)], 1u)] = p;
+        // exclusive scan of the 256 bucket counts (one entry per thread)
+        const uint32_t mine = hist[t];
+        for (uint32_t off = 1; off < SORT_BUCKETS; off <<= 1) {
+            const uint32_t add = t >= off ? hist[t - off] : 0u;
+            __syncthreads();
+            hist[t] += add;
+            __syncthreads();
+        }
+        hist[t] -= mine;
+        __syncthreads();
+        for (uint32_t i = t; i < n; i += WAVE_BLOCK) out[seg + atomicAdd(&hist[keys[i]], 1u)] = vals[i];
+        __syncthreads();
     }
 }
 
@@ -566,11 +561,7 @@ struct Launch {
 
 template <bool MULTI>
 void sort_queue(Wavefront& w, const Launch& L, uint32_t key, const uint32_t* in, const uint32_t* cnt, uint32_t* out) {
-    const hipStream_t stream = L.stream;
-    k_sort_clear<<<1, SORT_BUCKETS, 0, stream>>>(w.sort_hist);
-    k_sort_hist<MULTI><<<L.grid, WAVE_BLOCK, 0, stream>>>(key, *L.s0, L.scenes, L.jobs, w.ps, in, cnt, w.sort_hist);
-    k_sort_scan<<<1, SORT_BUCKETS, 0, stream>>>(w.sort_hist);
-    k_sort_scatter<MULTI><<<L.grid, WAVE_BLOCK, 0, stream>>>(key, *L.s0, L.scenes, L.jobs, w.ps, in, cnt, w.sort_hist, out);
+    k_sort_local<MULTI><<<L.grid, WAVE_BLOCK, 0, L.stream>>>(key, *L.s0, L.scenes, L.jobs, w.ps, in, cnt, out);
 }
 
 // One chunk: generate; per bounce b: trace (closest rays of b + shadow rays
