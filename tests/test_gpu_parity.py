@@ -137,3 +137,45 @@ def test_intersect_queries_match_oracle(ctx, field, field_scene):
     occ = field_scene.occluded(rays, tmax)
     rocc = oracle.occluded(field.desc, rays, tmax)
     assert np.array_equal(occ, rocc)
+
+
+ALPHA_MASK, ALPHA_BLEND = 1, 2
+
+
+@pytest.fixture(scope="module")
+def alpha_cornell(ctx):
+    # the Cornell box with alpha-tested materials: a stochastic (blend) tall box, a cut-out (mask,
+    # alpha < cutoff) back wall that shows the sky, and a mask short box that passes the cutoff
+    w = ptcore.World.generate(ptcore.SCENE_CORNELL, 1, 256, 256)
+    m = w.arrays()["materials"]
+    for k, mode, alpha in ((5, ALPHA_BLEND, 0.55), (2, ALPHA_MASK, 0.3), (6, ALPHA_MASK, 0.8)):
+        m["alpha_mode"][k] = mode
+        m["base_color"][k, 3] = alpha
+    assert (m["alpha_mode"][[2, 5, 6]] != 0).all()
+    s = ptcore.Scene(ctx, w.desc)
+    yield w, s
+    s.close()
+
+
+@pytest.mark.parametrize("sample", [1, 5])
+def test_alpha_primary_hits_bitexact(ctx, alpha_cornell, sample):
+    w, s = alpha_cornell
+    t = ptcore.Tracer(ctx, s, 256, 256)
+    tri, tt = t.primary_hits(sample)
+    ref_tri, ref_t = oracle.primary_hits(w.desc, 256, 256, sample)
+    assert np.array_equal(tri, ref_tri)
+    assert np.array_equal(tt[tri >= 0], ref_t[ref_tri >= 0])
+    t.close()
+
+
+@pytest.mark.parametrize("schedule", [ptcore.SCHEDULE_MEGAKERNEL, ptcore.SCHEDULE_WAVEFRONT])
+def test_alpha_radiance_cornell(ctx, alpha_cornell, schedule):
+    w, s = alpha_cornell
+    spp, bounces = 8, 8
+    t = ptcore.Tracer(ctx, s, 256, 256, schedule)
+    t.set_max_bounces(bounces)
+    t.trace(spp)
+    gpu = t.accumulation()
+    ref, _, _ = oracle.render_f32(w.desc, 256, 256, bounces, spp)
+    assert_parity(gpu, ref, f"alpha cornell schedule={schedule}")
+    t.close()

@@ -317,3 +317,25 @@ def test_watertight_shared_edges_and_vertices():
     _, bt = oracle.intersect_brute(world.desc, rays)
     t = oracle.intersect(world.desc, rays)[1]
     assert (t[m] >= bt[m]).all()
+
+
+@pytest.mark.parametrize("mode,alpha,cutoff,expect_pass", [(2, 0.3, 0.5, 0.7), (2, 0.85, 0.5, 0.15),
+                                                            (1, 0.3, 0.5, 1.0), (1, 0.7, 0.5, 0.0)])
+def test_alpha_coverage_closed_form(mode, alpha, cutoff, expect_pass):
+    # a black alpha-tested quad filling the view in front of a constant sky of luminance L:
+    # blend lets a ray through with probability 1 - alpha (trace.slang:426-454), mask iff alpha < cutoff,
+    # so the pixel's mean radiance at depth 0 is (pass probability) * L
+    L = 2.0
+    quad = [[-5, 0, -5], [5, 0, 5], [5, 0, -5]], [[-5, 0, -5], [-5, 0, 5], [5, 0, 5]]
+    geo, attr = ptcore.triangles_from_vertices([*quad], [0, 0])
+    mats = np.array([ptcore.default_material(base_color=(0, 0, 0, alpha), alpha_mode=mode, alpha_cutoff=cutoff,
+                                             specular_color_and_weight=(1, 1, 1, 0))])
+    sky = np.ones((4, 8, 4), np.float32)
+    world = ptcore.WorldInput(geo, attr, mats, sky, skybox_luminance=L, camera_position=(0.0, 0.5, 0.0),
+                              yaw=0.0, pitch=-90.0, fov=0.5, aspect=1.0).build()
+    n = 40_000
+    acc = oracle.render_f32(world.desc, 1, 1, 0, n)[0]
+    est = acc[0, 0, :3] / n
+    want = expect_pass * L
+    assert np.allclose(est, est[0])
+    assert abs(est[0] - want) <= 0.015 * L, (est, want)
