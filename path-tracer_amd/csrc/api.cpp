@@ -254,48 +254,65 @@ pt_status pt_scene_create(pt_context* c, const pt_scene_desc* d, pt_scene** out)
         opaque[i] = d->materials[d->attributes[i].material].alpha_mode == PT_ALPHA_MODE_OPAQUE ? 1 : 0;
         all_opaque = all_opaque && opaque[i];
     }
+    // traversal node copy with each node's child code precomputed (see DevScene::tnode)
+    std::vector<float4> tnodes(2 * (size_t)d->bvh_node_count);
+    for (uint32_t i = 0; i < d->bvh_node_count; ++i) {
+        const PtBvhNode& n = d->bvh_nodes[i];
+        uint32_t code = ptd::child_code(i, n.left_first, n.tri_count);
+        float cf, tf;
+        std::memcpy(&cf, &code, 4);
+        std::memcpy(&tf, &n.tri_count, 4);
+        tnodes[2 * (size_t)i] = make_float4(n.aabb_min[0], n.aabb_min[1], n.aabb_min[2], cf);
+        tnodes[2 * (size_t)i + 1] = make_float4(n.aabb_max[0], n.aabb_max[1], n.aabb_max[2], tf);
+    }
     std::vector<float4> sky((size_t)d->skybox.width * d->skybox.height);
     std::memcpy(sky.data(), d->skybox_rgba, sky.size() * sizeof(float4));
-    PtTriangleGeometry* geo;
-    PtTriangleAttribute* attr;
-    PtMaterial* mat;
-    PtBvhNode* bvh;
-    uint8_t *srgb, *lin, *opq;
-    float4* skyd;
-    PtSkyboxAliasEntry* sa;
-    PtSceneAliasEntry* ca;
+    PtTriangleGeometry* geo = nullptr;
+    PtTriangleAttribute* attr = nullptr;
+    PtMaterial* mat = nullptr;
+    PtBvhNode* bvh = nullptr;
+    float4* tn = nullptr;
+    uint8_t *srgb = nullptr, *lin = nullptr, *opq = nullptr;
+    float4* skyd = nullptr;
+    PtSkyboxAliasEntry* sa = nullptr;
+    PtSceneAliasEntry* ca = nullptr;
+    auto keep_pointers = [&] {  // the view owns whatever was allocated, so destroy frees it
+        v.geo = geo;
+        v.attr = attr;
+        v.mat = mat;
+        v.bvh = bvh;
+        v.tnode = tn;
+        v.srgb = srgb;
+        v.linear = lin;
+        v.tri_opaque = opq;
+        v.sky = skyd;
+        v.sky_alias = sa;
+        v.scene_alias = ca;
+    };
     if ((st = upload(&geo, d->geometries, d->triangle_count, fp, "geometries")) ||
         (st = upload(&attr, d->attributes, d->triangle_count, fp, "attributes")) ||
         (st = upload(&mat, d->materials, d->material_count, fp, "materials")) ||
         (st = upload(&bvh, d->bvh_nodes, d->bvh_node_count, fp, "bvh nodes")) ||
+        (st = upload(&tn, tnodes.data(), tnodes.size(), fp, "traversal nodes")) ||
         (st = upload(&srgb, d->srgb_atlases, layer * d->srgb_atlas_count, fp, "srgb atlases")) ||
         (st = upload(&lin, d->linear_atlases, layer * d->linear_atlas_count, fp, "linear atlases")) ||
         (st = upload(&opq, opaque.data(), opaque.size(), fp, "opacity")) ||
         (st = upload(&skyd, sky.data(), sky.size(), fp, "skybox")) ||
         (st = upload(&sa, d->skybox_alias, d->skybox_alias_count, fp, "skybox alias")) ||
         (st = upload(&ca, (const PtSceneAliasEntry*)nullptr, s->scene_alias_capacity, fp, "scene alias"))) {
-        v.geo = geo;  // so destroy frees what was allocated
+        keep_pointers();
         pt_scene_destroy(s);
         return st;
     }
     if (d->scene_alias_count) {
         hipError_t e = hipMemcpy(ca, d->scene_alias, d->scene_alias_count * sizeof(PtSceneAliasEntry), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
-            v.geo = geo;
+            keep_pointers();
             pt_scene_destroy(s);
             return hip_error(e, "scene alias");
         }
     }
-    v.geo = geo;
-    v.attr = attr;
-    v.mat = mat;
-    v.bvh = bvh;
-    v.srgb = srgb;
-    v.linear = lin;
-    v.tri_opaque = opq;
-    v.sky = skyd;
-    v.sky_alias = sa;
-    v.scene_alias = ca;
+    keep_pointers();
     v.srgb_lut = c->srgb_lut;
     v.camera = d->camera;
     v.skybox = d->skybox;
@@ -318,6 +335,7 @@ pt_status pt_scene_destroy(pt_scene* s) {
     hipFree((void*)v.attr);
     hipFree((void*)v.mat);
     hipFree((void*)v.bvh);
+    hipFree((void*)v.tnode);
     hipFree((void*)v.srgb);
     hipFree((void*)v.linear);
     hipFree((void*)v.tri_opaque);

@@ -305,8 +305,12 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_trace_refill(DevScene s0, const 
 // One bounce of shading for the paths in `queue` (trace.slang:886-1296).
 // Paths of scenes with alpha-tested materials test their shadow ray inline
 // (the any-hit draws random numbers between NEE and BSDF sampling).
+// minimum waves per SIMD the shade kernel is compiled for (its register budget)
+#ifndef PT_SHADE_WAVES_PER_EU
+#define PT_SHADE_WAVES_PER_EU 1
+#endif
 template <bool MULTI>
-__global__ __launch_bounds__(WAVE_BLOCK) void k_shade(DevScene s0, const DevScene* __restrict__ scenes,
+__global__ __launch_bounds__(WAVE_BLOCK, PT_SHADE_WAVES_PER_EU) void k_shade(DevScene s0, const DevScene* __restrict__ scenes,
                                                       const WaveJob* __restrict__ jobs, WaveJob j0, PathState ps,
                                                       uint32_t bounce, const uint32_t* __restrict__ queue,
                                                       const uint32_t* __restrict__ count_ptr,

@@ -26,6 +26,10 @@ struct DevScene {
     const PtTriangleAttribute* attr;
     const PtMaterial* mat;
     const PtBvhNode* bvh;
+    // traversal copy of the nodes: node i is tnode[2i] = (aabb_min, code),
+    // tnode[2i + 1] = (aabb_max, tri_count), with the code of child_code()
+    // precomputed, so a child pair read needs no decoding
+    const float4* tnode;
     const uint8_t* srgb;       // sRGB atlas layers, RGBA8
     const uint8_t* linear;     // linear atlas layers, RGBA8
     const float4* sky;         // normalised sky, RGBA32F
@@ -308,7 +312,7 @@ __device__ __forceinline__ float intersect_aabb(float4 lo, float4 hi, f3 neg, f3
 // ever re-read on pop, unlike trace.slang:793 which re-reads every node.
 constexpr uint32_t CODE_SMALL_LEAF = 0x80000000u;
 constexpr uint32_t CODE_BIG_LEAF = 0x40000000u;
-__device__ __forceinline__ uint32_t child_code(uint32_t node_index, uint32_t left_first, uint32_t tri_count) {
+__host__ __device__ __forceinline__ uint32_t child_code(uint32_t node_index, uint32_t left_first, uint32_t tri_count) {
     if (tri_count == 0u) return left_first;
     if (tri_count <= 2u) return CODE_SMALL_LEAF | (left_first << 1) | (tri_count - 1u);
     return CODE_BIG_LEAF | node_index;
@@ -353,9 +357,8 @@ __device__ __forceinline__ int32_t closest_hit(const DevScene& s, f3 o, f3 d, f3
     hit_v = 0.0f;
     hit_w = 0.0f;
     int32_t hit = -1;
-    const float4* __restrict__ nodes = reinterpret_cast<const float4*>(s.bvh);
-    PtBvhNode root = s.bvh[0];
-    uint32_t cur = child_code(0u, root.left_first, root.tri_count);
+    const float4* __restrict__ nodes = s.tnode;
+    uint32_t cur = __float_as_uint(nodes[0].w);
     st.sp = 0;
     for (;;) {
         if ((cur & (CODE_SMALL_LEAF | CODE_BIG_LEAF)) == 0u) {
@@ -366,8 +369,7 @@ __device__ __forceinline__ int32_t closest_hit(const DevScene& s, f3 o, f3 d, f3
             float dl = intersect_aabb(l0, l1, neg, rcp, hit_t);
             float dr = intersect_aabb(r0, r1, neg, rcp, hit_t);
             bool hl = dl >= 0.0f, hr = dr >= 0.0f;
-            uint32_t cl = child_code(cur, __float_as_uint(l0.w), __float_as_uint(l1.w));
-            uint32_t cr = child_code(cur + 1u, __float_as_uint(r0.w), __float_as_uint(r1.w));
+            const uint32_t cl = __float_as_uint(l0.w), cr = __float_as_uint(r0.w);
             if (hl && hr) {
                 bool left_closer = dl < dr;
                 st.push(left_closer ? cr : cl);
@@ -416,9 +418,8 @@ __device__ __forceinline__ bool occluded(const DevScene& s, f3 o, f3 d, float hi
     const f3 rcp = f3{1.0f, 1.0f, 1.0f} / d;
     const Watertight wt = watertight_setup(d, rcp);
     const f3 neg = (-o) * rcp;
-    const float4* __restrict__ nodes = reinterpret_cast<const float4*>(s.bvh);
-    PtBvhNode root = s.bvh[0];
-    uint32_t cur = child_code(0u, root.left_first, root.tri_count);
+    const float4* __restrict__ nodes = s.tnode;
+    uint32_t cur = __float_as_uint(nodes[0].w);
     st.sp = 0;
     for (;;) {
         if ((cur & (CODE_SMALL_LEAF | CODE_BIG_LEAF)) == 0u) {
@@ -428,8 +429,7 @@ __device__ __forceinline__ bool occluded(const DevScene& s, f3 o, f3 d, float hi
             float dl = intersect_aabb(l0, l1, neg, rcp, hit_t);
             float dr = intersect_aabb(r0, r1, neg, rcp, hit_t);
             bool hl = dl >= 0.0f, hr = dr >= 0.0f;
-            uint32_t cl = child_code(cur, __float_as_uint(l0.w), __float_as_uint(l1.w));
-            uint32_t cr = child_code(cur + 1u, __float_as_uint(r0.w), __float_as_uint(r1.w));
+            const uint32_t cl = __float_as_uint(l0.w), cr = __float_as_uint(r0.w);
             if (hl && hr) {
                 bool left_closer = dl < dr;
                 st.push(left_closer ? cr : cl);
@@ -496,15 +496,14 @@ __device__ __forceinline__ void ray_begin(const DevScene& s, RayState& r, f3 o, 
     r.hit = shadow ? 0 : -1;
     r.light = light;
     r.shadow = shadow;
-    PtBvhNode root = s.bvh[0];
-    r.cur = child_code(0u, root.left_first, root.tri_count);
+    r.cur = __float_as_uint(s.tnode[0].w);
     st.sp = 0;
 }
 
 // One traversal step; returns true when the ray is finished.
 template <bool COUNT>
 __device__ __forceinline__ bool ray_step(const DevScene& s, RayState& r, Rng& rng, Stack& st, TraversalCounters& ctr) {
-    const float4* __restrict__ nodes = reinterpret_cast<const float4*>(s.bvh);
+    const float4* __restrict__ nodes = s.tnode;
     if ((r.cur & (CODE_SMALL_LEAF | CODE_BIG_LEAF)) == 0u) {
         const uint32_t cur = r.cur;
         float4 l0 = nodes[2u * cur], l1 = nodes[2u * cur + 1u];
@@ -513,8 +512,7 @@ __device__ __forceinline__ bool ray_step(const DevScene& s, RayState& r, Rng& rn
         float dl = intersect_aabb(l0, l1, r.neg, r.rcp, r.hit_t);
         float dr = intersect_aabb(r0, r1, r.neg, r.rcp, r.hit_t);
         bool hl = dl >= 0.0f, hr = dr >= 0.0f;
-        uint32_t cl = child_code(cur, __float_as_uint(l0.w), __float_as_uint(l1.w));
-        uint32_t cr = child_code(cur + 1u, __float_as_uint(r0.w), __float_as_uint(r1.w));
+        const uint32_t cl = __float_as_uint(l0.w), cr = __float_as_uint(r0.w);
         if (hl && hr) {
             bool left_closer = dl < dr;
             st.push(left_closer ? cr : cl);

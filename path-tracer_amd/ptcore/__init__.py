@@ -22,7 +22,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libptcore.so")
+# PTCORE_LIB selects another build of the library (e.g. one made with different compile options)
+LIB_PATH = os.environ.get("PTCORE_LIB") or os.path.join(_HERE, "libptcore.so")
 
 
 class PtcoreUnavailable(RuntimeError):
