@@ -247,8 +247,9 @@ def main():
     ap.add_argument("--schedule", default="wavefront", choices=["wavefront", "mega"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=2)
-    ap.add_argument("--traversal", type=int, default=None, choices=[0, 1],
-                    help="wavefront traversal: 0 batch per wave, 1 lane refill (default: library default)")
+    ap.add_argument("--traversal", type=int, default=None, choices=[0, 1, 2],
+                    help="wavefront traversal: 0 batch per wave, 1 lane refill, 2 threshold refill "
+                         "(default: library default)")
     ap.add_argument("--sort", type=int, default=None, choices=[0, 1, 2], help="queue sort key")
     ap.add_argument("--lds", type=int, default=None, choices=[16, 24, 32], help="LDS stack depth")
     args = ap.parse_args()

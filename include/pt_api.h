@@ -230,8 +230,13 @@ typedef struct pt_wave_config {
     uint32_t chunk_paths;      /* paths in flight per chunk (the queue tile size) */
     uint32_t sort_key;         /* ray-queue order: 0 none, 1 direction octant, 2 hit material */
     uint32_t traversal;        /* 0: each thread traces its rays one batch of the wave at a time;
-                                  1: lane refill (a lane that finishes starts its next ray at once) */
+                                  1: lane refill (a lane that finishes starts its next ray at once);
+                                  2: threshold refill (idle lanes start their next rays together
+                                     once fewer than half of the wave is still traversing) */
 } pt_wave_config;
+#define PT_TRAVERSAL_BATCH 0
+#define PT_TRAVERSAL_REFILL 1
+#define PT_TRAVERSAL_FETCH 2
 #define PT_SORT_NONE 0
 #define PT_SORT_OCTANT 1
 #define PT_SORT_MATERIAL 2

@@ -35,7 +35,7 @@ pt_status check_config(const pt_wave_config* c) {
     if (c->chunk_paths < 4096 || c->chunk_paths > (1u << 27))
         return set_error(PT_ERR_INVALID, "chunk_paths must be in [4096, 2^27]");
     if (c->sort_key > PT_SORT_MATERIAL) return set_error(PT_ERR_INVALID, "unknown sort key %u", c->sort_key);
-    if (c->traversal > 1) return set_error(PT_ERR_INVALID, "unknown traversal mode %u", c->traversal);
+    if (c->traversal > 2) return set_error(PT_ERR_INVALID, "unknown traversal mode %u", c->traversal);
     return PT_OK;
 }
 
@@ -126,10 +126,10 @@ pt_status pt_autotune_batch(pt_tracer* const* tracers, uint32_t count, const uin
     const uint32_t lds_opts[] = {16, 24, 32};
     const uint32_t chunk_opts[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
     const uint32_t sort_opts[] = {ptd::SORT_NONE, ptd::SORT_OCTANT, ptd::SORT_MATERIAL};
-    const uint32_t traversal_opts[] = {0, 1};
+    const uint32_t traversal_opts[] = {ptd::TRAVERSAL_BATCH, ptd::TRAVERSAL_REFILL, ptd::TRAVERSAL_FETCH};
     for (int knob = 0; knob < 4; ++knob) {
         const uint32_t* opts = knob == 0 ? traversal_opts : knob == 1 ? chunk_opts : (knob == 2 ? sort_opts : lds_opts);
-        const int nopts = knob == 0 ? 2 : (knob == 1 ? 4 : 3);
+        const int nopts = knob == 0 ? 3 : (knob == 1 ? 4 : 3);
         ptd::WaveConfig base = best;
         for (int k = 0; k < nopts; ++k) {
             ptd::WaveConfig cand = base;

@@ -30,8 +30,9 @@ struct WaveConfig {
     uint32_t lds_depth = 16;          // traversal stack entries in LDS (16/24/32)
     uint32_t chunk_paths = 1u << 24;  // paths in flight per chunk (queue tile size)
     uint32_t sort_key = SORT_NONE;    // ray-queue ordering
-    uint32_t traversal = 0;           // 0: one ray per thread per batch, 1: lane refill
+    uint32_t traversal = 0;           // a PT_TRAVERSAL_* mode
 };
+enum Traversal : uint32_t { TRAVERSAL_BATCH = 0, TRAVERSAL_REFILL = 1, TRAVERSAL_FETCH = 2 };
 
 }  // namespace ptd
 

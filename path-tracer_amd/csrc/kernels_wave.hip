@@ -27,6 +27,21 @@
 namespace ptd {
 
 constexpr uint32_t WAVE_BLOCK = 256;
+// Traversal launches: one wave per block, many more waves than a CU holds at
+// once (TRACE_WAVES_PER_CU; 32 are resident at full occupancy), so the
+// hardware dispatcher keeps handing waves to SIMDs whose waves finish early
+// instead of leaving them idle behind the slowest ray batch of the grid.
+#ifndef PT_TRACE_BLOCK
+#define PT_TRACE_BLOCK 64u
+#endif
+#ifndef PT_TRACE_WAVES_PER_CU
+#define PT_TRACE_WAVES_PER_CU 256u
+#endif
+// blocks per CU of the shade launches
+#ifndef PT_SHADE_BLOCKS_PER_CU
+#define PT_SHADE_BLOCKS_PER_CU 32u
+#endif
+constexpr uint32_t TRACE_BLOCK = PT_TRACE_BLOCK;
 constexpr uint32_t MAX_COUNTS = 2 * (PT_MAX_BOUNCES_LIMIT + 2);
 constexpr uint32_t SORT_BUCKETS = 256;
 
@@ -172,25 +187,25 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_generate(DevScene s0, WaveJob j0
 // path's radiance here, after that bounce's emission (shade b) and before the
 // next emission (shade b + 1), which is the shader's order.
 template <uint32_t LDS_DEPTH, bool COUNT, bool MULTI>
-__global__ __launch_bounds__(WAVE_BLOCK) void k_trace(DevScene s0, const DevScene* __restrict__ scenes,
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene s0, const DevScene* __restrict__ scenes,
                                                       const WaveJob* __restrict__ jobs, PathState ps,
                                                       const uint32_t* __restrict__ closest_queue,
                                                       const uint32_t* __restrict__ closest_count,
                                                       const uint32_t* __restrict__ shadow_queue,
                                                       const uint32_t* __restrict__ shadow_count,
                                                       uint32_t* __restrict__ ovf, unsigned long long* __restrict__ counters) {
-    __shared__ uint32_t lds_stack[LDS_DEPTH * WAVE_BLOCK];
+    __shared__ uint32_t lds_stack[LDS_DEPTH * TRACE_BLOCK];
     Stack st;
     st.lds = lds_stack + threadIdx.x;
-    st.lds_stride = WAVE_BLOCK;
+    st.lds_stride = TRACE_BLOCK;
     st.lds_depth = LDS_DEPTH;
     st.ovf = ovf;
-    st.ovf_stride = gridDim.x * WAVE_BLOCK;
-    st.gtid = blockIdx.x * WAVE_BLOCK + threadIdx.x;
+    st.ovf_stride = gridDim.x * TRACE_BLOCK;
+    st.gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     const uint32_t nc = closest_queue ? *closest_count : 0u;
     const uint32_t ns = shadow_queue ? *shadow_count : 0u;
     TraversalCounters cc = {0, 0}, sc = {0, 0};
-    for (uint32_t j = blockIdx.x * WAVE_BLOCK + threadIdx.x; j < nc + ns; j += gridDim.x * WAVE_BLOCK) {
+    for (uint32_t j = blockIdx.x * TRACE_BLOCK + threadIdx.x; j < nc + ns; j += gridDim.x * TRACE_BLOCK) {
         if (j < nc) {
             const uint32_t p = closest_queue[j];
             const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
@@ -229,7 +244,7 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_trace(DevScene s0, const DevScen
 // next ray (grid-stride order) in the same iteration, so a wave runs for about
 // the average ray cost of its lanes instead of its slowest ray per batch.
 template <uint32_t LDS_DEPTH, bool COUNT, bool MULTI>
-__global__ __launch_bounds__(WAVE_BLOCK) void k_trace_refill(DevScene s0, const DevScene* __restrict__ scenes,
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_refill(DevScene s0, const DevScene* __restrict__ scenes,
                                                              const WaveJob* __restrict__ jobs, PathState ps,
                                                              const uint32_t* __restrict__ closest_queue,
                                                              const uint32_t* __restrict__ closest_count,
@@ -237,20 +252,20 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_trace_refill(DevScene s0, const 
                                                              const uint32_t* __restrict__ shadow_count,
                                                              uint32_t* __restrict__ ovf,
                                                              unsigned long long* __restrict__ counters) {
-    __shared__ uint32_t lds_stack[LDS_DEPTH * WAVE_BLOCK];
+    __shared__ uint32_t lds_stack[LDS_DEPTH * TRACE_BLOCK];
     Stack st;
     st.lds = lds_stack + threadIdx.x;
-    st.lds_stride = WAVE_BLOCK;
+    st.lds_stride = TRACE_BLOCK;
     st.lds_depth = LDS_DEPTH;
     st.ovf = ovf;
-    st.ovf_stride = gridDim.x * WAVE_BLOCK;
-    st.gtid = blockIdx.x * WAVE_BLOCK + threadIdx.x;
+    st.ovf_stride = gridDim.x * TRACE_BLOCK;
+    st.gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     const uint32_t nc = closest_queue ? *closest_count : 0u;
     const uint32_t ns = shadow_queue ? *shadow_count : 0u;
     const uint32_t total = nc + ns;
-    const uint32_t stride = gridDim.x * WAVE_BLOCK;
+    const uint32_t stride = gridDim.x * TRACE_BLOCK;
     TraversalCounters cc = {0, 0}, sc = {0, 0};
-    uint32_t j = blockIdx.x * WAVE_BLOCK + threadIdx.x;
+    uint32_t j = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     RayState r;
     uint32_t p = 0;
     Rng rng{0u};
@@ -302,14 +317,105 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_trace_refill(DevScene s0, const 
     }
 }
 
+// The same work as k_trace, one phase for the closest rays and one for the
+// shadow rays, with threshold refill: a lane whose ray is finished idles until
+// fewer than PT_REFILL_MIN_ACTIVE lanes of its wave are still traversing; then
+// every idle lane starts its next ray (grid-stride order) together, so the
+// setup of new rays runs once for many lanes and waves stay mostly full.
+#ifndef PT_REFILL_MIN_ACTIVE
+#define PT_REFILL_MIN_ACTIVE 32
+#endif
+template <bool SHADOW, bool COUNT, bool MULTI>
+__device__ __forceinline__ void trace_phase(const DevScene& s0, const DevScene* __restrict__ scenes,
+                                            const WaveJob* __restrict__ jobs, const PathState& ps,
+                                            const uint32_t* __restrict__ queue, uint32_t n, Stack& st,
+                                            TraversalCounters& ctr) {
+    const uint32_t stride = gridDim.x * TRACE_BLOCK;
+    uint32_t j = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    bool have = false, opaque = true;
+    uint32_t p = 0;
+    RayState r;
+    Rng rng{0u};
+    for (;;) {
+        if (__popcll(__ballot(have)) < PT_REFILL_MIN_ACTIVE) {
+            if (!have && j < n) {
+                p = queue[j];
+                j += stride;
+                have = true;
+                const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
+                if (SHADOW) {
+                    // only opaque scenes queue shadow rays: any-hit draws nothing
+                    ray_begin(s, r, ld3(ps.sox, ps.soy, ps.soz, p), ld3(ps.sdx, ps.sdy, ps.sdz, p), true, ps.sdist[p],
+                              ps.slight[p], st);
+                } else {
+                    opaque = s.all_opaque != 0u;
+                    rng.s = opaque ? 0u : ps.rng[p];
+                    ray_begin(s, r, ld3(ps.ox, ps.oy, ps.oz, p), ld3(ps.dx, ps.dy, ps.dz, p), false, 0.0f, -1, st);
+                }
+            }
+            if (__ballot(have) == 0ull) break;
+        }
+        if (have) {
+            const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
+            if (ray_advance<SHADOW, COUNT>(s, r, rng, st, ctr)) {
+                have = false;
+                if (SHADOW) {
+                    if (r.hit == 0) {
+                        f3 L = ld3(ps.lr, ps.lg, ps.lb, p) + ld3(ps.scr, ps.scg, ps.scb, p);
+                        st3(ps.lr, ps.lg, ps.lb, p, L);
+                    }
+                } else {
+                    ps.hit[p] = r.hit;
+                    ps.ht[p] = r.hit_t;
+                    ps.hv[p] = r.hit_v;
+                    ps.hw[p] = r.hit_w;
+                    if (!opaque) ps.rng[p] = rng.s;
+                }
+            }
+        }
+    }
+}
+
+template <uint32_t LDS_DEPTH, bool COUNT, bool MULTI>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_fetch(DevScene s0, const DevScene* __restrict__ scenes,
+                                                            const WaveJob* __restrict__ jobs, PathState ps,
+                                                            const uint32_t* __restrict__ closest_queue,
+                                                            const uint32_t* __restrict__ closest_count,
+                                                            const uint32_t* __restrict__ shadow_queue,
+                                                            const uint32_t* __restrict__ shadow_count,
+                                                            uint32_t* __restrict__ ovf,
+                                                            unsigned long long* __restrict__ counters) {
+    __shared__ uint32_t lds_stack[LDS_DEPTH * TRACE_BLOCK];
+    Stack st;
+    st.lds = lds_stack + threadIdx.x;
+    st.lds_stride = TRACE_BLOCK;
+    st.lds_depth = LDS_DEPTH;
+    st.ovf = ovf;
+    st.ovf_stride = gridDim.x * TRACE_BLOCK;
+    st.gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    TraversalCounters cc = {0, 0}, sc = {0, 0};
+    if (closest_queue)
+        trace_phase<false, COUNT, MULTI>(s0, scenes, jobs, ps, closest_queue, *closest_count, st, cc);
+    if (shadow_queue)
+        trace_phase<true, COUNT, MULTI>(s0, scenes, jobs, ps, shadow_queue, *shadow_count, st, sc);
+    if (COUNT) {
+        atomicAdd(&counters[2], (unsigned long long)cc.pairs);
+        atomicAdd(&counters[3], (unsigned long long)cc.tris);
+        atomicAdd(&counters[4], (unsigned long long)sc.pairs);
+        atomicAdd(&counters[5], (unsigned long long)sc.tris);
+    }
+}
+
 // One bounce of shading for the paths in `queue` (trace.slang:886-1296).
 // Paths of scenes with alpha-tested materials test their shadow ray inline
-// (the any-hit draws random numbers between NEE and BSDF sampling).
+// (the any-hit draws random numbers between NEE and BSDF sampling); only the
+// INLINE_SHADOW instance carries that traversal and its LDS stack, so launches
+// over opaque scenes keep the smaller register and LDS footprint.
 // minimum waves per SIMD the shade kernel is compiled for (its register budget)
 #ifndef PT_SHADE_WAVES_PER_EU
 #define PT_SHADE_WAVES_PER_EU 1
 #endif
-template <bool MULTI>
+template <bool MULTI, bool INLINE_SHADOW>
 __global__ __launch_bounds__(WAVE_BLOCK, PT_SHADE_WAVES_PER_EU) void k_shade(DevScene s0, const DevScene* __restrict__ scenes,
                                                       const WaveJob* __restrict__ jobs, WaveJob j0, PathState ps,
                                                       uint32_t bounce, const uint32_t* __restrict__ queue,
@@ -317,9 +423,9 @@ __global__ __launch_bounds__(WAVE_BLOCK, PT_SHADE_WAVES_PER_EU) void k_shade(Dev
                                                       uint32_t* __restrict__ next_queue, uint32_t* __restrict__ next_count,
                                                       uint32_t* __restrict__ shadow_queue, uint32_t* __restrict__ shadow_count,
                                                       uint32_t* __restrict__ ovf) {
-    __shared__ uint32_t lds_stack[MEGA_LDS_DEPTH * WAVE_BLOCK];
+    __shared__ uint32_t lds_stack[INLINE_SHADOW ? MEGA_LDS_DEPTH * WAVE_BLOCK : 1];
     Stack st;
-    st.lds = lds_stack + threadIdx.x;
+    st.lds = lds_stack + (INLINE_SHADOW ? threadIdx.x : 0u);
     st.lds_stride = WAVE_BLOCK;
     st.lds_depth = MEGA_LDS_DEPTH;
     st.ovf = ovf;
@@ -368,7 +474,7 @@ __global__ __launch_bounds__(WAVE_BLOCK, PT_SHADE_WAVES_PER_EU) void k_shade(Dev
                     evaluate_lobes(b, sf.wo, lwi, f, pdf);
                     const float weight = bounce < max_bounces ? power_heuristic(light.pdf, pdf) : 1.0f;
                     const f3 contrib = throughput * f * lwi.z * light.emission * (weight / light.pdf);
-                    if (!s.all_opaque) {
+                    if (INLINE_SHADOW && !s.all_opaque) {
                         TraversalCounters ctr = {0, 0};
                         if (!occluded<false>(s, next_origin, light.direction, light.distance, light.triangle_index, rng, st, ctr))
                             L = L + contrib;
@@ -536,12 +642,18 @@ template <bool COUNT, bool MULTI>
 void launch_trace(uint32_t lds, uint32_t refill, uint32_t grid, hipStream_t st, const DevScene& s0,
                   const DevScene* scenes, const WaveJob* jobs, const PathState& ps, const uint32_t* cq,
                   const uint32_t* cc, const uint32_t* sq, const uint32_t* scnt, uint32_t* ovf, unsigned long long* ctr) {
-#define PT_TRACE_LAUNCH(K, D) K<D, COUNT, MULTI><<<grid, WAVE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, cq, cc, sq, scnt, ovf, ctr)
-    if (refill) {
+#define PT_TRACE_LAUNCH(K, D) K<D, COUNT, MULTI><<<grid, TRACE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, cq, cc, sq, scnt, ovf, ctr)
+    if (refill == TRAVERSAL_REFILL) {
         switch (lds) {
             case 24: PT_TRACE_LAUNCH(k_trace_refill, 24); break;
             case 32: PT_TRACE_LAUNCH(k_trace_refill, 32); break;
             default: PT_TRACE_LAUNCH(k_trace_refill, 16); break;
+        }
+    } else if (refill == TRAVERSAL_FETCH) {
+        switch (lds) {
+            case 24: PT_TRACE_LAUNCH(k_trace_fetch, 24); break;
+            case 32: PT_TRACE_LAUNCH(k_trace_fetch, 32); break;
+            default: PT_TRACE_LAUNCH(k_trace_fetch, 16); break;
         }
     } else {
         switch (lds) {
@@ -555,12 +667,15 @@ void launch_trace(uint32_t lds, uint32_t refill, uint32_t grid, hipStream_t st, 
 
 struct Launch {
     uint32_t grid;
+    uint32_t trace_grid;  // blocks of the traversal launches
+    uint32_t shade_grid;  // blocks of the shade launches
     hipStream_t stream;
     const DevScene* s0;
     const DevScene* scenes;
     const WaveJob* jobs;
     const WaveJob* j0;
     ChunkUnits units;
+    bool any_alpha;  // some scene has alpha-tested materials: shade tests those shadow rays inline
 };
 
 template <bool MULTI>
@@ -588,10 +703,10 @@ void run_chunk(Wavefront& w, const Launch& L, uint32_t max_bounces, bool countin
     auto trace = [&](const uint32_t* cq, const uint32_t* cc, const uint32_t* sq, const uint32_t* sc) {
         EventScope tm(lg, EV_EXTEND, stream);
         if (counting)
-            launch_trace<true, MULTI>(w.cfg.lds_depth, w.cfg.traversal, grid, stream, *L.s0, L.scenes, L.jobs, ps, cq, cc, sq,
+            launch_trace<true, MULTI>(w.cfg.lds_depth, w.cfg.traversal, L.trace_grid, stream, *L.s0, L.scenes, L.jobs, ps, cq, cc, sq,
                                       sc, w.ovf, ctr);
         else
-            launch_trace<false, MULTI>(w.cfg.lds_depth, w.cfg.traversal, grid, stream, *L.s0, L.scenes, L.jobs, ps, cq, cc,
+            launch_trace<false, MULTI>(w.cfg.lds_depth, w.cfg.traversal, L.trace_grid, stream, *L.s0, L.scenes, L.jobs, ps, cq, cc,
                                        sq, sc, w.ovf, ctr);
     };
     for (uint32_t b = 0; b <= max_bounces; ++b) {
@@ -607,8 +722,12 @@ void run_chunk(Wavefront& w, const Launch& L, uint32_t max_bounces, bool countin
         }
         {
             EventScope tm(lg, EV_SHADE, stream);
-            k_shade<MULTI><<<grid, WAVE_BLOCK, 0, stream>>>(*L.s0, L.scenes, L.jobs, *L.j0, ps, b, cur, cnt, nxt,
-                                                            w.counts + 2 * (b + 1), shadow, cnt + 1, w.ovf);
+            if (L.any_alpha)
+                k_shade<MULTI, true><<<L.shade_grid, WAVE_BLOCK, 0, stream>>>(*L.s0, L.scenes, L.jobs, *L.j0, ps, b, cur, cnt, nxt,
+                                                                      w.counts + 2 * (b + 1), shadow, cnt + 1, w.ovf);
+            else
+                k_shade<MULTI, false><<<L.shade_grid, WAVE_BLOCK, 0, stream>>>(*L.s0, L.scenes, L.jobs, *L.j0, ps, b, cur, cnt,
+                                                                       nxt, w.counts + 2 * (b + 1), shadow, cnt + 1, w.ovf);
         }
         std::swap(cur, nxt);
     }
@@ -720,13 +839,15 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
         if (h.block_begin > be || be > tile_blocks(j)) return set_error(PT_ERR_INVALID, "job block range out of the tile");
         total_units += be - h.block_begin;
     }
-    (void)any_alpha;
     const uint32_t chunk_cap = std::max(cfg.chunk_paths, max_unit_paths);
     pt_status st = reserve(chunk_cap);
     if (st) return st;
     const uint32_t grid = ctx->cu_count * 8u;
+    const uint32_t trace_grid = ctx->cu_count * PT_TRACE_WAVES_PER_CU / (TRACE_BLOCK / 64u);
+    const uint32_t shade_grid = ctx->cu_count * PT_SHADE_BLOCKS_PER_CU;
     const uint32_t lds_min = std::min(cfg.lds_depth, MEGA_LDS_DEPTH);
-    const size_t need = (size_t)(max_depth > lds_min ? max_depth - lds_min : 1u) * grid * WAVE_BLOCK * 4u;
+    const size_t need = (size_t)(max_depth > lds_min ? max_depth - lds_min : 1u) *
+                        std::max<size_t>((size_t)shade_grid * WAVE_BLOCK, (size_t)trace_grid * TRACE_BLOCK) * 4u;
     if (need > ovf_bytes) {
         if (ovf) {
             hipStreamSynchronize(stream);
@@ -833,6 +954,8 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
     for (size_t c = 0; c < chunks.size(); ++c) {
         Launch L;
         L.grid = grid;
+        L.trace_grid = trace_grid;
+        L.shade_grid = shade_grid;
         L.stream = stream;
         L.s0 = &scene_tab[0];
         L.j0 = &job_tab[0];
@@ -843,6 +966,7 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
         L.units.offset = d_uoff + off_start[c];
         L.units.n_units = (uint32_t)(chunks[c].unit_end - chunks[c].unit_begin);
         L.units.n_paths = chunks[c].paths;
+        L.any_alpha = any_alpha;
         if (multi) run_chunk<true>(*this, L, max_bounces, counting, lg, ctr);
         else run_chunk<false>(*this, L, max_bounces, counting, lg, ctr);
         e = hipGetLastError();

@@ -558,6 +558,67 @@ __device__ __forceinline__ bool ray_step(const DevScene& s, RayState& r, Rng& rn
     return false;
 }
 
+// One traversal round for the refill schedule: walk internal nodes until a
+// leaf is current (the while-while form), test that leaf, pop. Returns true
+// when the ray is finished. The node order, tests and acceptance are those of
+// closest_hit (SHADOW false) and occluded (SHADOW true), so results are equal.
+template <bool SHADOW, bool COUNT>
+__device__ __forceinline__ bool ray_advance(const DevScene& s, RayState& r, Rng& rng, Stack& st, TraversalCounters& ctr) {
+    const float4* __restrict__ nodes = s.tnode;
+    uint32_t cur = r.cur;
+    while ((cur & (CODE_SMALL_LEAF | CODE_BIG_LEAF)) == 0u) {
+        float4 l0 = nodes[2u * cur], l1 = nodes[2u * cur + 1u];
+        float4 r0 = nodes[2u * cur + 2u], r1 = nodes[2u * cur + 3u];
+        if (COUNT) ++ctr.pairs;
+        float dl = intersect_aabb(l0, l1, r.neg, r.rcp, r.hit_t);
+        float dr = intersect_aabb(r0, r1, r.neg, r.rcp, r.hit_t);
+        bool hl = dl >= 0.0f, hr = dr >= 0.0f;
+        const uint32_t cl = __float_as_uint(l0.w), cr = __float_as_uint(r0.w);
+        if (hl && hr) {
+            bool left_closer = dl < dr;
+            st.push(left_closer ? cr : cl);
+            cur = left_closer ? cl : cr;
+        } else if (hl || hr) {
+            cur = hl ? cl : cr;
+        } else {
+            if (st.sp == 0u) return true;
+            cur = st.pop();
+        }
+    }
+    uint32_t first, count;
+    if (cur & CODE_SMALL_LEAF) {
+        first = (cur >> 1) & 0x3fffffffu;
+        count = (cur & 1u) + 1u;
+    } else {
+        PtBvhNode n = s.bvh[cur & 0x3fffffffu];
+        first = n.left_first;
+        count = n.tri_count;
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t ti = first + i;
+        if (SHADOW && (int32_t)ti == r.light) continue;
+        if (COUNT) ++ctr.tris;
+        float tt, bv, bw;
+        if (!tri_test(s.geo[ti].v, r.o, r.wt, r.hit_t, tt, bv, bw)) continue;
+        if (SHADOW) {
+            if (!passes_through(s, ti, bv, bw, rng)) {
+                r.hit = 1;
+                st.sp = 0;
+                return true;
+            }
+        } else {
+            if (passes_through(s, ti, bv, bw, rng)) continue;
+            r.hit_t = tt;
+            r.hit = (int32_t)ti;
+            r.hit_v = bv;
+            r.hit_w = bw;
+        }
+    }
+    if (st.sp == 0u) return true;
+    r.cur = st.pop();
+    return false;
+}
+
 // ------------------------------------------------------------------ camera ray (trace.slang:736-748)
 __device__ __forceinline__ f3 primary_direction(const DevScene& s, uint32_t px, uint32_t py, uint32_t iw, uint32_t ih,
                                                 Rng& rng) {
