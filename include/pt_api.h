@@ -120,8 +120,67 @@ typedef struct pt_scene_desc {
     uint32_t scene_alias_count;
 } pt_scene_desc;
 
-/* Borrow the prepared arrays of `world` (valid until pt_world_destroy). */
+/* Borrow the prepared arrays of `world` (valid until pt_world_destroy or the
+ * next pt_world_take_updates, whose edits may replace the alias table). */
 pt_status pt_world_scene_desc(const pt_world* world, pt_scene_desc* out);
+
+/* ------------------------------------------------------- world editing */
+/* Camera parameters (world.rs:360-376): angles in radians, pitch in
+ * [-pi/2, pi/2], vertical fov in [0, pi). */
+typedef struct pt_camera_params {
+    float position[3];
+    float yaw;
+    float pitch;
+    float fov;
+    float aspect_ratio;
+} pt_camera_params;
+
+/* Current (initial == 0) or initial camera (World::camera / initial_camera,
+ * world.rs:700-706). Edits made since the last take_updates are not included. */
+pt_status pt_world_camera(const pt_world* world, int initial, pt_camera_params* out);
+/* World::edit_camera (world.rs:708-716): replaces the pending camera. */
+pt_status pt_world_edit_camera(pt_world* world, const pt_camera_params* camera);
+/* Current and initial material `index` (World::material, world.rs:725-734);
+ * either output may be null. */
+pt_status pt_world_material(const pt_world* world, uint32_t index, PtMaterial* current, PtMaterial* initial);
+/* World::edit_material (world.rs:736-783): the fields before alpha_mode are
+ * taken from `candidate` when all of them are finite (*accepted = 1), else
+ * the edit is ignored (*accepted = 0); alpha mode, cutoff and texture
+ * bindings stay as they are. */
+pt_status pt_world_edit_material(pt_world* world, uint32_t index, const PtMaterial* candidate, int* accepted);
+/* World::edit_emission_scale (world.rs:654-690): every emissive material's
+ * emission luminance becomes its initial one times max(scale, 0). */
+pt_status pt_world_edit_emission_scale(pt_world* world, float scale);
+pt_status pt_world_emission_scale(const pt_world* world, float* scale);
+/* World::edit_skybox_average_luminance (world.rs:644-648), in nits (>= 0). */
+pt_status pt_world_edit_skybox_luminance(pt_world* world, float luminance);
+/* Scene::pick_material (world.rs:1595-1688): the material of the triangle a
+ * ray from `position` along the camera's forward axis hits first, or -1. */
+pt_status pt_world_pick_material(const pt_world* world, const float position[3], float yaw, float pitch,
+                                 int32_t* material);
+
+/* What changed since the last take (WorldUpdates, world.rs:73-101). Arrays
+ * point into the world and stay valid until its next take or destroy. */
+typedef struct pt_world_updates {
+    uint32_t camera_changed;
+    PtCamera camera;
+    uint32_t material_count;               /* edited materials, ascending index */
+    const uint32_t* material_indices;
+    const PtMaterial* materials;
+    uint32_t scene_uniforms_changed;
+    PtSceneUniforms scene_uniforms;
+    uint32_t scene_alias_changed;          /* emissive set or emission changed */
+    const PtSceneAliasEntry* scene_alias;
+    uint32_t scene_alias_count;
+    uint32_t skybox_uniforms_changed;
+    PtSkyboxUniforms skybox_uniforms;
+    uint32_t reset_accumulation;
+    int32_t selected_material;             /* pick_material at the camera, -1 for none */
+} pt_world_updates;
+/* World::take_updates (world.rs:785-884): applies the pending edits to the
+ * world, rebuilding the emissive alias table and the light probabilities when
+ * emission or sky luminance changed, and reports what a device scene needs. */
+pt_status pt_world_take_updates(pt_world* world, pt_world_updates* out);
 
 /* ------------------------------------------------------------ device scene */
 typedef struct pt_scene pt_scene;
@@ -141,6 +200,10 @@ pt_status pt_scene_update_materials(pt_scene* scene, const uint32_t* indices,
 pt_status pt_scene_update_lights(pt_scene* scene, const PtSceneUniforms* uniforms,
                                  const PtSceneAliasEntry* alias, uint32_t alias_count,
                                  const PtSkyboxUniforms* skybox);
+/* PathTracer::apply_updates (path_tracer.rs:272-282) for what
+ * pt_world_take_updates reported: writes the changed parts to the device and
+ * resets tracers' accumulation only when reset_accumulation is set. */
+pt_status pt_scene_apply_updates(pt_scene* scene, const pt_world_updates* updates);
 
 /* Closest-hit queries on the device BVH for arbitrary rays (host arrays).
  * rays: count * 6 floats (origin xyz, direction xyz). Outputs per ray: hit

@@ -244,6 +244,7 @@ pt_status pt_scene_create(pt_context* c, const pt_scene_desc* d, pt_scene** out)
     // like the reference, room for an alias entry per triangle so light
     // updates never reallocate (world_resources.rs:158-165)
     s->scene_alias_capacity = std::max(d->scene_alias_count, d->triangle_count);
+    s->alias_count = d->scene_alias_count;
     size_t fp = 0;
     ptd::DevScene& v = s->view;
     std::memset(&v, 0, sizeof(v));
@@ -377,33 +378,86 @@ pt_status pt_scene_update_materials(pt_scene* s, const uint32_t* idx, const PtMa
     return PT_OK;
 }
 
-pt_status pt_scene_update_lights(pt_scene* s, const PtSceneUniforms* u, const PtSceneAliasEntry* alias, uint32_t n,
-                                 const PtSkyboxUniforms* sky) {
-    if (!s || !u) return set_error(PT_ERR_INVALID, "null argument");
+// Checks that light uniforms and an alias table of n entries (null: the one
+// on the device stays, with `n` its entry count) are consistent with the scene.
+static pt_status validate_lights(const pt_scene* s, const PtSceneUniforms* u, const PtSceneAliasEntry* alias, uint32_t n) {
     if (u->emissive_triangles_count > n || n > s->scene_alias_capacity)
         return set_error(PT_ERR_INVALID, "alias table size %u does not fit (capacity %u, emissive %u)", n,
                          s->scene_alias_capacity, u->emissive_triangles_count);
-    for (uint32_t i = 0; i < u->emissive_triangles_count; ++i)
-        if (alias[i].alias_index >= u->emissive_triangles_count || alias[i].triangle_index >= s->triangle_count)
-            return set_error(PT_ERR_INVALID, "scene alias entry %u out of range", i);
+    if (alias)
+        for (uint32_t i = 0; i < u->emissive_triangles_count; ++i)
+            if (alias[i].alias_index >= u->emissive_triangles_count || alias[i].triangle_index >= s->triangle_count)
+                return set_error(PT_ERR_INVALID, "scene alias entry %u out of range", i);
     const float pg = u->light_probabilities[0], ps = u->light_probabilities[1];
     if ((ps > 0.0f || (pg + ps > 0.0f && pg < 1.0f)) && s->skybox_alias_count == 0)
         return set_error(PT_ERR_INVALID, "sky sampling without a sky alias table");
     if (pg > 0.0f && u->emissive_triangles_count == 0)
         return set_error(PT_ERR_INVALID, "geometry light probability without emitters");
+    return PT_OK;
+}
+
+// Copies n alias entries to the device in stream order (after queued renders).
+static pt_status write_alias(pt_scene* s, const PtSceneAliasEntry* alias, uint32_t n) {
+    if (!n) return PT_OK;
+    hipError_t e = hipMemcpyAsync((void*)s->view.scene_alias, alias, n * sizeof(PtSceneAliasEntry),
+                                  hipMemcpyHostToDevice, s->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+    return e == hipSuccess ? PT_OK : hip_error(e, "alias update");
+}
+
+static void set_skybox_uniforms(pt_scene* s, const PtSkyboxUniforms& sky) {
+    PtSkyboxUniforms k = sky;
+    k.width = s->view.skybox.width;  // the sky image itself is not replaced
+    k.height = s->view.skybox.height;
+    s->view.skybox = k;
+}
+
+pt_status pt_scene_update_lights(pt_scene* s, const PtSceneUniforms* u, const PtSceneAliasEntry* alias, uint32_t n,
+                                 const PtSkyboxUniforms* sky) {
+    if (!s || !u || (n && !alias)) return set_error(PT_ERR_INVALID, "null argument");
+    pt_status st = validate_lights(s, u, alias, n);
+    if (st) return st;
     hipSetDevice(s->ctx->device);
-    if (n) {
-        hipError_t e = hipMemcpy((void*)s->view.scene_alias, alias, n * sizeof(PtSceneAliasEntry), hipMemcpyHostToDevice);
-        if (e != hipSuccess) return hip_error(e, "alias update");
-    }
+    if ((st = write_alias(s, alias, n))) return st;
     s->view.uniforms = *u;
-    if (sky) {
-        PtSkyboxUniforms k = *sky;
-        k.width = s->view.skybox.width;
-        k.height = s->view.skybox.height;
-        s->view.skybox = k;
-    }
+    s->alias_count = n;
+    if (sky) set_skybox_uniforms(s, *sky);
     ++s->version;
+    return PT_OK;
+}
+
+pt_status pt_scene_apply_updates(pt_scene* s, const pt_world_updates* u) {
+    if (!s || !u) return set_error(PT_ERR_INVALID, "null argument");
+    if (u->material_count && (!u->material_indices || !u->materials))
+        return set_error(PT_ERR_INVALID, "material updates without arrays");
+    if (u->scene_alias_changed && u->scene_alias_count && !u->scene_alias)
+        return set_error(PT_ERR_INVALID, "alias update without entries");
+    for (uint32_t i = 0; i < u->material_count; ++i) {
+        if (u->material_indices[i] >= s->material_count)
+            return set_error(PT_ERR_INVALID, "material index %u out of range", u->material_indices[i]);
+        if (u->materials[i].alpha_mode != PT_ALPHA_MODE_OPAQUE && s->view.all_opaque)
+            return set_error(PT_ERR_INVALID, "switching a material to alpha-tested needs a new scene");
+    }
+    const PtSceneUniforms& uni = u->scene_uniforms_changed ? u->scene_uniforms : s->view.uniforms;
+    const uint32_t n = u->scene_alias_changed ? u->scene_alias_count : s->alias_count;
+    if (u->scene_uniforms_changed || u->scene_alias_changed) {
+        pt_status st = validate_lights(s, &uni, u->scene_alias_changed ? u->scene_alias : nullptr, n);
+        if (st) return st;
+    }
+    hipSetDevice(s->ctx->device);
+    if (u->material_count) {
+        pt_status st = pt_scene_update_materials(s, u->material_indices, u->materials, u->material_count);
+        if (st) return st;
+    }
+    if (u->scene_alias_changed) {
+        pt_status st = write_alias(s, u->scene_alias, u->scene_alias_count);
+        if (st) return st;
+        s->alias_count = u->scene_alias_count;
+    }
+    if (u->scene_uniforms_changed) s->view.uniforms = u->scene_uniforms;
+    if (u->skybox_uniforms_changed) set_skybox_uniforms(s, u->skybox_uniforms);
+    if (u->camera_changed) s->view.camera = u->camera;
+    if (u->reset_accumulation) ++s->version;
     return PT_OK;
 }
 

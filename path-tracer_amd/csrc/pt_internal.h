@@ -3,6 +3,7 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <map>
 #include <vector>
 
 #include "../../include/pt_api.h"
@@ -26,11 +27,24 @@ struct World {
     PtSceneUniforms scene_uniforms{};
     std::vector<PtSceneAliasEntry> scene_alias;
     PtCamera camera{};
+    // editing state (World, world.rs:53-71 and 589-884)
+    pt_camera_params cam{}, initial_cam{};
+    std::vector<PtMaterial> initial_materials;
+    double radius_squared = 0.0;
+    float emission_scale = 1.0f;
+    bool pending_camera = false, pending_luminance = false;
+    pt_camera_params pending_cam{};
+    float pending_lum = 0.0f;
+    std::map<uint32_t, PtMaterial> pending_materials;  // ascending index, like the reference's BTreeMap
+    std::vector<uint32_t> update_indices;              // arrays of the last take_updates
+    std::vector<PtMaterial> update_materials;
+    int32_t selected_material = -1;
 };
 
 std::vector<PtSkyboxAliasEntry> alias_table(const std::vector<double>& weights);
 void update_light_probabilities(PtSceneUniforms& u, float skybox_flux);
 PtCamera gpu_camera(const float pos[3], float yaw, float pitch, float fov, float aspect);
+int32_t pick_material(const World& w, const float pos[3], float yaw, float pitch);
 
 }  // namespace ptcore
 

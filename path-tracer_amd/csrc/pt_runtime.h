@@ -35,6 +35,7 @@ struct Scene {
     uint32_t bvh_depth = 0;
     uint32_t triangle_count = 0, material_count = 0, bvh_node_count = 0;
     uint32_t scene_alias_capacity = 0, skybox_alias_count = 0;
+    uint32_t alias_count = 0;  // scene alias entries currently on the device
     uint64_t footprint = 0;
     uint64_t version = 0;  // bumped by every update; tracers reset accumulation on change
 };

@@ -17,6 +17,7 @@
 // Floating-point steps follow the reference's types (f32 with mul_add where
 // it uses mul_add, f64 where it widens), so the arrays come out bit-identical.
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -299,6 +300,87 @@ PtCamera gpu_camera(const float pos[3], float yaw, float pitch, float fov, float
     return c;
 }
 
+// Scene::pick_material (world.rs:1595-1688): its own traversal (boxes tested
+// at pop, left child popped first, no distance ordering) and its own
+// triangle-distance expression, so the pick equals the reference's.
+int32_t pick_material(const World& w, const float pos[3], float yaw, float pitch) {
+    float sy = sinf(yaw), cy = cosf(yaw), sp = sinf(pitch), cp = cosf(pitch);
+    float ry[3][3] = {{cy, 0.0f, -sy}, {0.0f, 1.0f, 0.0f}, {sy, 0.0f, cy}};
+    float rx[3][3] = {{1.0f, 0.0f, 0.0f}, {0.0f, cp, sp}, {0.0f, -sp, cp}};
+    auto mulv = [](const float m[3][3], const float v[3], float out[3]) {
+        for (int k = 0; k < 3; ++k) out[k] = (m[0][k] * v[0] + m[1][k] * v[1]) + m[2][k] * v[2];
+    };
+    float rot[3][3];
+    for (int c = 0; c < 3; ++c) mulv(ry, rx[c], rot[c]);
+    const float NZ[3] = {0.0f, 0.0f, -1.0f};
+    float d[3], rcp[3], ad[3];
+    mulv(rot, NZ, d);
+    for (int k = 0; k < 3; ++k) {
+        rcp[k] = 1.0f / d[k];
+        ad[k] = std::fabs(d[k]);
+    }
+    const int kxy = ad[0] <= ad[1] ? 1 : 0;
+    const int kz = ad[kxy] > ad[2] ? kxy : 2;
+    const int kx_val = kz == 2 ? 0 : kz + 1;
+    const int ky_val = kx_val == 2 ? 0 : kx_val + 1;
+    const bool is_neg = d[kz] < 0.0f;
+    const int kx = is_neg ? ky_val : kx_val, ky = is_neg ? kx_val : ky_val;
+    const float sz = rcp[kz], sx = -d[kx] * sz, syy = -d[ky] * sz;
+    std::vector<uint32_t> stack{0u};
+    float closest_t = std::numeric_limits<float>::max();
+    int64_t hit = -1;
+    while (!stack.empty()) {
+        const PtBvhNode& node = w.bvh_nodes[stack.back()];
+        stack.pop_back();
+        float t_near = -INFINITY, t_far = INFINITY;
+        for (int k = 0; k < 3; ++k) {
+            float t0 = (node.aabb_min[k] - pos[k]) * rcp[k], t1 = (node.aabb_max[k] - pos[k]) * rcp[k];
+            float lo = std::fmin(t0, t1), hi = std::fmax(t0, t1);
+            t_near = k == 0 ? lo : std::fmax(t_near, lo);
+            t_far = k == 0 ? hi : std::fmin(t_far, hi);
+        }
+        if (!(t_near <= t_far && t_far > 0.0f && t_near < closest_t)) continue;
+        if (node.tri_count > 0) {
+            for (uint32_t i = 0; i < node.tri_count; ++i) {
+                const uint32_t ti = node.left_first + i;
+                const float* v = w.geometries[ti].v;
+                float a[3], b[3], c[3];
+                for (int k = 0; k < 3; ++k) {
+                    a[k] = v[k] - pos[k];
+                    b[k] = v[3 + k] - pos[k];
+                    c[k] = v[6 + k] - pos[k];
+                }
+                float ax = std::fmaf(sx, a[kz], a[kx]), ay = std::fmaf(syy, a[kz], a[ky]);
+                float bx = std::fmaf(sx, b[kz], b[kx]), by = std::fmaf(syy, b[kz], b[ky]);
+                float cx = std::fmaf(sx, c[kz], c[kx]), cyy = std::fmaf(syy, c[kz], c[ky]);
+                float u = std::fmaf(cyy, -bx, cx * by);
+                float vv = std::fmaf(ay, -cx, ax * cyy);
+                float ww = std::fmaf(by, -ax, bx * ay);
+                if (std::fmin(u, std::fmin(vv, ww)) < 0.0f && std::fmax(u, std::fmax(vv, ww)) > 0.0f) continue;
+                float det = u + vv + ww;
+                if (det == 0.0f) continue;
+                float t = std::fmaf(ww * sz, c[kz], std::fmaf(u * sz, a[kz], vv * sz * b[kz])) / det;
+                if (t > 0.0f && t < closest_t) {
+                    closest_t = t;
+                    hit = ti;
+                }
+            }
+        } else {
+            stack.push_back(node.left_first + 1u);
+            stack.push_back(node.left_first);
+        }
+    }
+    return hit < 0 ? -1 : (int32_t)w.attributes[(size_t)hit].material;
+}
+
+// Skybox::update_total_luminous_flux (world.rs:1977-1987)
+void update_sky_flux(World& w) {
+    w.skybox.total_luminous_flux =
+        w.sky_alias.empty() ? 0.0f
+                            : (float)std::min(4.0 * (M_PI * M_PI) * w.radius_squared * (double)w.skybox.average_luminance,
+                                              (double)std::numeric_limits<float>::max());
+}
+
 }  // namespace ptcore
 
 using namespace ptcore;
@@ -403,12 +485,8 @@ pt_status pt_world_build(const pt_world_desc* d, pt_world** out) {
         const PtBvhNode& root = w->bvh_nodes[0];
         double dx = (double)root.aabb_max[0] - root.aabb_min[0], dy = (double)root.aabb_max[1] - root.aabb_min[1],
                dz = (double)root.aabb_max[2] - root.aabb_min[2];
-        double radius_squared = ((dx * dx + dy * dy) + dz * dz) * 0.25;
-        w->skybox.total_luminous_flux =
-            w->sky_alias.empty()
-                ? 0.0f
-                : (float)std::min(4.0 * (M_PI * M_PI) * radius_squared * (double)w->skybox.average_luminance,
-                                  (double)std::numeric_limits<float>::max());
+        w->radius_squared = ((dx * dx + dy * dy) + dz * dz) * 0.25;
+        update_sky_flux(*w);
         update_light_probabilities(w->scene_uniforms, w->skybox.total_luminous_flux);
 
         // camera: Camera::new from the root box, then the headless options (headless.rs:193-202)
@@ -416,6 +494,7 @@ pt_status pt_world_build(const pt_world_desc* d, pt_world** out) {
         const float* mx = root.aabb_max;
         float pos[3] = {(mn[0] + mx[0]) * 0.5f, (mn[1] + mx[1]) * 0.5f,
                         std::fmaf(std::fmax(mx[0] - mn[0], mx[1] - mn[1]), 0.6f, mx[2])};
+        for (int k = 0; k < 3; ++k) w->initial_cam.position[k] = pos[k];
         if (!std::isnan(d->camera_position[0]))
             for (int k = 0; k < 3; ++k) pos[k] = d->camera_position[k];
         const float rads_per_deg = 3.14159265358979323846f / 180.0f;
@@ -429,7 +508,19 @@ pt_status pt_world_build(const pt_world_desc* d, pt_world** out) {
             fov = std::fmin(std::fmax(fov, 0.0f), 3.14159265358979323846f - std::numeric_limits<float>::epsilon());
         }
         float aspect = (std::isfinite(d->aspect_ratio) && d->aspect_ratio > 0.0f) ? d->aspect_ratio : 1.0f;
+        // the initial camera is Camera::new resized to the aspect ratio (world.rs:596-597)
+        w->initial_cam.yaw = 0.0f;
+        w->initial_cam.pitch = 0.0f;
+        w->initial_cam.fov = half_pi;
+        w->initial_cam.aspect_ratio = aspect;
         w->camera = gpu_camera(pos, yaw, pitch, fov, aspect);
+        for (int k = 0; k < 3; ++k) w->cam.position[k] = pos[k];
+        w->cam.yaw = yaw;
+        w->cam.pitch = pitch;
+        w->cam.fov = fov;
+        w->cam.aspect_ratio = aspect;
+        w->initial_materials = w->materials;
+        w->selected_material = pick_material(*w, pos, yaw, pitch);
     } catch (const std::bad_alloc&) {
         delete w;
         return set_error(PT_ERR_MEMORY, "out of host memory while building the world");
@@ -467,6 +558,139 @@ pt_status pt_world_scene_desc(const pt_world* wp, pt_scene_desc* s) {
     s->scene_uniforms = w.scene_uniforms;
     s->scene_alias = w.scene_alias.empty() ? nullptr : w.scene_alias.data();
     s->scene_alias_count = (uint32_t)w.scene_alias.size();
+    return PT_OK;
+}
+
+pt_status pt_world_camera(const pt_world* w, int initial, pt_camera_params* out) {
+    if (!w || !out) return set_error(PT_ERR_INVALID, "pt_world_camera: null argument");
+    *out = initial ? w->initial_cam : w->cam;
+    return PT_OK;
+}
+
+pt_status pt_world_edit_camera(pt_world* w, const pt_camera_params* c) {
+    if (!w || !c) return set_error(PT_ERR_INVALID, "pt_world_edit_camera: null argument");
+    w->pending_camera = true;
+    w->pending_cam = *c;
+    return PT_OK;
+}
+
+pt_status pt_world_material(const pt_world* w, uint32_t index, PtMaterial* current, PtMaterial* initial) {
+    if (!w) return set_error(PT_ERR_INVALID, "pt_world_material: null argument");
+    if (index >= w->materials.size()) return set_error(PT_ERR_INVALID, "material %u out of range", index);
+    if (current) *current = w->materials[index];
+    if (initial) *initial = w->initial_materials[index];
+    return PT_OK;
+}
+
+pt_status pt_world_edit_material(pt_world* w, uint32_t index, const PtMaterial* candidate, int* accepted) {
+    if (!w || !candidate) return set_error(PT_ERR_INVALID, "pt_world_edit_material: null argument");
+    if (accepted) *accepted = 0;
+    if (index >= w->materials.size()) return PT_OK;  // the reference ignores unknown materials
+    auto it = w->pending_materials.find(index);
+    const PtMaterial original = it != w->pending_materials.end() ? it->second : w->materials[index];
+    // the editable fields are the f32 prefix before alpha_mode
+    constexpr size_t editable = offsetof(PtMaterial, alpha_mode);
+    float vals[editable / sizeof(float)];
+    std::memcpy(vals, candidate, editable);
+    for (float v : vals)
+        if (!std::isfinite(v)) return PT_OK;
+    PtMaterial acc = original;
+    std::memcpy(&acc, candidate, editable);
+    if (std::memcmp(&acc, &w->materials[index], sizeof(PtMaterial)) == 0) w->pending_materials.erase(index);
+    else w->pending_materials[index] = acc;
+    if (accepted) *accepted = 1;
+    return PT_OK;
+}
+
+pt_status pt_world_edit_emission_scale(pt_world* w, float scale) {
+    if (!w) return set_error(PT_ERR_INVALID, "pt_world_edit_emission_scale: null argument");
+    scale = std::fmax(scale, 0.0f);
+    w->emission_scale = scale;
+    for (uint32_t i = 0; i < w->materials.size(); ++i) {
+        auto it = w->pending_materials.find(i);
+        PtMaterial e = it != w->pending_materials.end() ? it->second : w->materials[i];
+        // emissive: luminance(emission colour) * emission weight > 0 (world.rs:676-681)
+        if (luminance(e.emission_color_and_luminance) * (double)e.emission_weight <= 0.0) continue;
+        e.emission_color_and_luminance[3] = w->initial_materials[i].emission_color_and_luminance[3] * scale;
+        if (std::memcmp(&e, &w->materials[i], sizeof(PtMaterial)) == 0) w->pending_materials.erase(i);
+        else w->pending_materials[i] = e;
+    }
+    return PT_OK;
+}
+
+pt_status pt_world_emission_scale(const pt_world* w, float* scale) {
+    if (!w || !scale) return set_error(PT_ERR_INVALID, "pt_world_emission_scale: null argument");
+    *scale = w->emission_scale;
+    return PT_OK;
+}
+
+pt_status pt_world_edit_skybox_luminance(pt_world* w, float luminance) {
+    if (!w) return set_error(PT_ERR_INVALID, "pt_world_edit_skybox_luminance: null argument");
+    w->pending_luminance = true;
+    w->pending_lum = std::fmax(luminance, 0.0f);
+    return PT_OK;
+}
+
+pt_status pt_world_pick_material(const pt_world* w, const float position[3], float yaw, float pitch,
+                                 int32_t* material) {
+    if (!w || !position || !material) return set_error(PT_ERR_INVALID, "pt_world_pick_material: null argument");
+    *material = pick_material(*w, position, yaw, pitch);
+    return PT_OK;
+}
+
+pt_status pt_world_take_updates(pt_world* w, pt_world_updates* out) {
+    if (!w || !out) return set_error(PT_ERR_INVALID, "pt_world_take_updates: null argument");
+    std::memset(out, 0, sizeof(*out));
+    bool camera_changed = false, selection_changed = false;
+    if (w->pending_camera) {
+        const pt_camera_params& c = w->pending_cam;
+        selection_changed = c.position[0] != w->cam.position[0] || c.position[1] != w->cam.position[1] ||
+                            c.position[2] != w->cam.position[2] || c.yaw != w->cam.yaw || c.pitch != w->cam.pitch;
+        camera_changed = selection_changed || c.fov != w->cam.fov || c.aspect_ratio != w->cam.aspect_ratio;
+        w->cam = c;
+        w->pending_camera = false;
+    }
+    bool luminance_changed = false;
+    if (w->pending_luminance) {
+        luminance_changed = w->pending_lum != w->skybox.average_luminance;
+        w->skybox.average_luminance = w->pending_lum;
+        w->pending_luminance = false;
+    }
+    bool lighting_changed = false;
+    w->update_indices.clear();
+    w->update_materials.clear();
+    for (const auto& [index, m] : w->pending_materials) {
+        PtMaterial& old = w->materials[index];
+        lighting_changed = lighting_changed || old.emission_weight != m.emission_weight ||
+                           std::memcmp(old.emission_color_and_luminance, m.emission_color_and_luminance,
+                                       sizeof(m.emission_color_and_luminance)) != 0;
+        old = m;
+        w->update_indices.push_back(index);
+        w->update_materials.push_back(m);
+    }
+    w->pending_materials.clear();
+    if (lighting_changed) scene_lights(*w, w->scene_uniforms, w->scene_alias);
+    const bool skybox_uniforms_changed = luminance_changed;
+    if (skybox_uniforms_changed) update_sky_flux(*w);
+    const bool scene_uniforms_changed = lighting_changed || skybox_uniforms_changed;
+    if (scene_uniforms_changed) update_light_probabilities(w->scene_uniforms, w->skybox.total_luminous_flux);
+    if (camera_changed) w->camera = gpu_camera(w->cam.position, w->cam.yaw, w->cam.pitch, w->cam.fov, w->cam.aspect_ratio);
+    if (selection_changed) w->selected_material = pick_material(*w, w->cam.position, w->cam.yaw, w->cam.pitch);
+
+    out->camera_changed = camera_changed;
+    out->camera = w->camera;
+    out->material_count = (uint32_t)w->update_indices.size();
+    out->material_indices = w->update_indices.empty() ? nullptr : w->update_indices.data();
+    out->materials = w->update_materials.empty() ? nullptr : w->update_materials.data();
+    out->scene_uniforms_changed = scene_uniforms_changed;
+    out->scene_uniforms = w->scene_uniforms;
+    out->scene_alias_changed = lighting_changed;
+    out->scene_alias = w->scene_alias.empty() ? nullptr : w->scene_alias.data();
+    out->scene_alias_count = (uint32_t)w->scene_alias.size();
+    out->skybox_uniforms_changed = skybox_uniforms_changed;
+    out->skybox_uniforms = w->skybox;
+    out->reset_accumulation = camera_changed || luminance_changed || !w->update_indices.empty();
+    out->selected_material = w->selected_material;
     return PT_OK;
 }
 

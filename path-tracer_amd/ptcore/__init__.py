@@ -156,7 +156,22 @@ class ServerStats(C.Structure):
                 ("jobs_rejected", C.c_uint64), ("busy_s", C.c_double), ("bytes_in_use", C.c_uint64)]
 
 
+class CameraParams(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("yaw", C.c_float), ("pitch", C.c_float), ("fov", C.c_float),
+                ("aspect_ratio", C.c_float)]
+
+
+class WorldUpdates(C.Structure):
+    _fields_ = [("camera_changed", C.c_uint32), ("camera", PtCamera),
+                ("material_count", C.c_uint32), ("material_indices", _p), ("materials", _p),
+                ("scene_uniforms_changed", C.c_uint32), ("scene_uniforms", PtSceneUniforms),
+                ("scene_alias_changed", C.c_uint32), ("scene_alias", _p), ("scene_alias_count", C.c_uint32),
+                ("skybox_uniforms_changed", C.c_uint32), ("skybox_uniforms", PtSkyboxUniforms),
+                ("reset_accumulation", C.c_uint32), ("selected_material", C.c_int32)]
+
+
 SORT_NONE, SORT_OCTANT, SORT_MATERIAL = 0, 1, 2
+TRAVERSAL_BATCH, TRAVERSAL_REFILL, TRAVERSAL_FETCH = 0, 1, 2
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
 
 # every symbol include/pt_api.h and include/pt_scenes.h declare: (name, restype, argtypes)
@@ -170,12 +185,22 @@ _SIGNATURES = [
     ("pt_world_build", _i32, [C.POINTER(WorldDesc), C.POINTER(_p)]),
     ("pt_world_destroy", _i32, [_p]),
     ("pt_world_scene_desc", _i32, [_p, C.POINTER(SceneDesc)]),
+    ("pt_world_camera", _i32, [_p, C.c_int, C.POINTER(CameraParams)]),
+    ("pt_world_edit_camera", _i32, [_p, C.POINTER(CameraParams)]),
+    ("pt_world_material", _i32, [_p, _u32, _p, _p]),
+    ("pt_world_edit_material", _i32, [_p, _u32, _p, C.POINTER(C.c_int)]),
+    ("pt_world_edit_emission_scale", _i32, [_p, _f32]),
+    ("pt_world_emission_scale", _i32, [_p, C.POINTER(_f32)]),
+    ("pt_world_edit_skybox_luminance", _i32, [_p, _f32]),
+    ("pt_world_pick_material", _i32, [_p, C.POINTER(_f32), _f32, _f32, C.POINTER(_i32)]),
+    ("pt_world_take_updates", _i32, [_p, C.POINTER(WorldUpdates)]),
     ("pt_scene_create", _i32, [_p, C.POINTER(SceneDesc), C.POINTER(_p)]),
     ("pt_scene_destroy", _i32, [_p]),
     ("pt_scene_footprint_bytes", _i32, [_p, C.POINTER(C.c_uint64)]),
     ("pt_scene_update_camera", _i32, [_p, C.POINTER(PtCamera)]),
     ("pt_scene_update_materials", _i32, [_p, _p, _p, _u32]),
     ("pt_scene_update_lights", _i32, [_p, C.POINTER(PtSceneUniforms), _p, _u32, C.POINTER(PtSkyboxUniforms)]),
+    ("pt_scene_apply_updates", _i32, [_p, C.POINTER(WorldUpdates)]),
     ("pt_scene_intersect", _i32, [_p, _p, _u32, _p, _p, _p, _p, _p, _p]),
     ("pt_scene_occluded", _i32, [_p, _p, _p, _p, _u32, _p]),
     ("pt_tracer_create", _i32, [_p, _p, _u32, _u32, C.POINTER(_p)]),
@@ -302,6 +327,63 @@ class World:
         g = Generated(kind, seed, width, height, triangles)
         w = cls(g.desc, keep=g)
         return w
+
+    # ---- editing (World, world.rs:589-884)
+    def camera(self, initial=False):
+        c = CameraParams()
+        _check(lib().pt_world_camera(self.handle, 1 if initial else 0, C.byref(c)))
+        return dict(position=tuple(c.position), yaw=c.yaw, pitch=c.pitch, fov=c.fov, aspect_ratio=c.aspect_ratio)
+
+    def edit_camera(self, **fields):
+        cur = self.camera()
+        cur.update(fields)
+        c = CameraParams((C.c_float * 3)(*cur["position"]), cur["yaw"], cur["pitch"], cur["fov"], cur["aspect_ratio"])
+        _check(lib().pt_world_edit_camera(self.handle, C.byref(c)))
+
+    def material(self, index):
+        cur, ini = np.zeros(1, MATERIAL_DTYPE), np.zeros(1, MATERIAL_DTYPE)
+        _check(lib().pt_world_material(self.handle, index, _ptr(cur), _ptr(ini)))
+        return cur[0], ini[0]
+
+    def edit_material(self, index, material):
+        m = np.ascontiguousarray(np.asarray(material, dtype=MATERIAL_DTYPE).reshape(1))
+        ok = C.c_int()
+        _check(lib().pt_world_edit_material(self.handle, index, _ptr(m), C.byref(ok)))
+        return bool(ok.value)
+
+    def edit_emission_scale(self, scale):
+        _check(lib().pt_world_edit_emission_scale(self.handle, float(scale)))
+
+    def emission_scale(self):
+        v = C.c_float()
+        _check(lib().pt_world_emission_scale(self.handle, C.byref(v)))
+        return v.value
+
+    def edit_skybox_luminance(self, nits):
+        _check(lib().pt_world_edit_skybox_luminance(self.handle, float(nits)))
+
+    def pick_material(self, position, yaw, pitch):
+        pos = (C.c_float * 3)(*position)
+        m = C.c_int32()
+        _check(lib().pt_world_pick_material(self.handle, pos, float(yaw), float(pitch), C.byref(m)))
+        return m.value
+
+    def take_updates(self):
+        """pt_world_take_updates: the raw struct (valid until the next take) plus numpy copies."""
+        u = WorldUpdates()
+        _check(lib().pt_world_take_updates(self.handle, C.byref(u)))
+        # the edits change the arrays a scene is created from: borrow them again
+        _check(lib().pt_world_scene_desc(self.handle, C.byref(self.desc)))
+        n = u.material_count
+        return {"raw": u, "camera_changed": bool(u.camera_changed), "camera": u.camera,
+                "material_indices": _view(u.material_indices, np.uint32, n).copy() if n else np.zeros(0, np.uint32),
+                "materials": _view(u.materials, MATERIAL_DTYPE, n).copy() if n else np.zeros(0, MATERIAL_DTYPE),
+                "scene_uniforms_changed": bool(u.scene_uniforms_changed), "scene_uniforms": u.scene_uniforms,
+                "scene_alias_changed": bool(u.scene_alias_changed),
+                "scene_alias": (_view(u.scene_alias, SCENE_ALIAS_DTYPE, u.scene_alias_count).copy()
+                                if u.scene_alias_count else np.zeros(0, SCENE_ALIAS_DTYPE)),
+                "skybox_uniforms_changed": bool(u.skybox_uniforms_changed), "skybox_uniforms": u.skybox_uniforms,
+                "reset_accumulation": bool(u.reset_accumulation), "selected_material": u.selected_material}
 
     def close(self):
         if self.handle:
@@ -485,6 +567,10 @@ class Scene:
 
     def update_camera(self, camera: PtCamera):
         _check(lib().pt_scene_update_camera(self.handle, C.byref(camera)))
+
+    def apply_updates(self, updates):
+        """pt_scene_apply_updates with what World.take_updates returned (call before the next take)."""
+        _check(lib().pt_scene_apply_updates(self.handle, C.byref(updates["raw"])))
 
     def update_materials(self, indices, materials):
         idx = np.ascontiguousarray(indices, dtype=np.uint32)
