@@ -46,19 +46,26 @@ def percentile(xs, q):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
-def cpu_baseline(world, w, h, bounces, samples):
+def cpu_baseline(world, w, h, bounces, samples, repeats):
     """The CPU oracle (C++ restatement of the reference method, OpenMP over
-    pixels) on the host cores: full frame at `samples` spp."""
+    pixels) on the host cores: the benchmark's whole job (full frame at
+    `samples` spp), `repeats` times; the median run is reported."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import binding as oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    t0 = time.perf_counter()
-    _, st = oracle.render_rows_f32(world.desc, w, h, bounces, samples, 0, h, threads=threads)
-    dt = time.perf_counter() - t0
+    runs = []
+    for _ in range(max(1, repeats)):
+        t0 = time.perf_counter()
+        _, st = oracle.render_rows_f32(world.desc, w, h, bounces, samples, 0, h, threads=threads)
+        runs.append((time.perf_counter() - t0, st))
+    runs.sort(key=lambda r: r[0])
+    dt, st = runs[len(runs) // 2]
+    total = sum(r[0] for r in runs)
     return {"value": w * h * samples / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
-            "sample": f"full {w}x{h} frame at {samples} spp (of the benchmark's spp), depth {bounces}, "
-                      f"{dt:.1f} s wall; rays/s {((st['closest_rays'] + st['shadow_rays']) / dt):.4g}"}
+            "sample": f"the benchmark job ({w}x{h}, {samples} spp, depth {bounces}) run {len(runs)} times, "
+                      f"median {dt:.2f} s of {total:.1f} s total; rays/s "
+                      f"{((st['closest_rays'] + st['shadow_rays']) / dt):.4g}"}
 
 
 def load_traffic(name):
@@ -246,7 +253,7 @@ def main():
     ap.add_argument("--jobs-per-rate", type=int, default=40)
     ap.add_argument("--schedule", default="wavefront", choices=["wavefront", "mega"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-spp", type=int, default=2)
+    ap.add_argument("--cpu-repeats", type=int, default=5, help="CPU baseline runs of the whole job (median)")
     ap.add_argument("--traversal", type=int, default=None, choices=[0, 1, 2],
                     help="wavefront traversal: 0 batch per wave, 1 lane refill, 2 threshold refill "
                          "(default: library default)")
@@ -377,7 +384,7 @@ def main():
                                                                    "tri_tests", "shadow_pair_tests", "shadow_tri_tests")}
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (oracle on host cores)...", file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline(world, W, H, BOUNCES, min(args.cpu_spp, SPP))
+        result["cpu_baseline"] = cpu_baseline(world, W, H, BOUNCES, SPP, args.cpu_repeats)
     else:
         result["cpu_baseline"] = None
     tracer.close()

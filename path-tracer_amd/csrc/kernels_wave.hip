@@ -49,6 +49,18 @@ __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(0xffffffffu, __builtin_amdgcn_mbcnt_lo(0xffffffffu, 0u));
 }
 
+// Adds the traversal counters of a wave's lanes to counters[2..5] with one
+// atomic per counter per wave (per-lane atomics on four addresses serialise).
+__device__ __forceinline__ void add_counters(unsigned long long* counters, const TraversalCounters& cc,
+                                             const TraversalCounters& sc) {
+    unsigned long long v[4] = {cc.pairs, cc.tris, sc.pairs, sc.tris};
+    for (int k = 0; k < 4; ++k)
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+    if (lane_id() == 0u)
+        for (int k = 0; k < 4; ++k)
+            if (v[k]) atomicAdd(&counters[2 + k], v[k]);
+}
+
 // Block-level staging of queue appends: lanes append to an LDS buffer
 // (wave-aggregated LDS atomic); the block moves the buffer to the global queue
 // with one global atomic when it could overflow on the next iteration and at
@@ -159,19 +171,15 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_generate(DevScene s0, WaveJob j0
             block_pixel(units.block[u] * 64u + lane, jb.tile_w, gx, gy);
             const uint32_t px = gx + jb.origin_x, py = gy + jb.origin_y;
             active = gx < jb.tile_w && gy < jb.tile_h && px < jb.image_w && py < jb.image_h;
-            ps.lr[i] = 0.0f;
-            ps.lg[i] = 0.0f;
-            ps.lb[i] = 0.0f;
+            // a camera ray: its origin, the unit throughput, the zero radiance
+            // and the unused previous pdf are implied at bounce 0, not stored
             if (active) {
                 const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, ji);
                 Rng rng{ptm::rng_seed(px, py, jb.first_sample + k)};
                 f3 d = primary_direction(s, px, py, jb.image_w, jb.image_h, rng);
-                st3(ps.ox, ps.oy, ps.oz, i, f3{s.camera.position[0], s.camera.position[1], s.camera.position[2]});
                 st3(ps.dx, ps.dy, ps.dz, i, d);
-                st3(ps.tr, ps.tg, ps.tb, i, f3{1.0f, 1.0f, 1.0f});
-                ps.prev_pdf[i] = 0.0f;
                 ps.rng[i] = rng.s;
-                ps.job[i] = ji;
+                if (MULTI) ps.job[i] = ji;
             }
         }
         bq_push(bq, active, i);
@@ -190,7 +198,7 @@ template <uint32_t LDS_DEPTH, bool COUNT, bool MULTI>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene s0, const DevScene* __restrict__ scenes,
                                                       const WaveJob* __restrict__ jobs, PathState ps,
                                                       const uint32_t* __restrict__ closest_queue,
-                                                      const uint32_t* __restrict__ closest_count,
+                                                      const uint32_t* __restrict__ closest_count, uint32_t first,
                                                       const uint32_t* __restrict__ shadow_queue,
                                                       const uint32_t* __restrict__ shadow_count,
                                                       uint32_t* __restrict__ ovf, unsigned long long* __restrict__ counters) {
@@ -209,7 +217,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene s0, const DevSce
         if (j < nc) {
             const uint32_t p = closest_queue[j];
             const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
-            const f3 o = ld3(ps.ox, ps.oy, ps.oz, p), d = ld3(ps.dx, ps.dy, ps.dz, p);
+            const f3 o = first ? camera_origin(s) : ld3(ps.ox, ps.oy, ps.oz, p), d = ld3(ps.dx, ps.dy, ps.dz, p);
             const f3 rcp = f3{1.0f, 1.0f, 1.0f} / d;
             const bool opaque = s.all_opaque != 0u;
             Rng rng{opaque ? 0u : ps.rng[p]};
@@ -232,10 +240,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene s0, const DevSce
         }
     }
     if (COUNT) {
-        atomicAdd(&counters[2], (unsigned long long)cc.pairs);
-        atomicAdd(&counters[3], (unsigned long long)cc.tris);
-        atomicAdd(&counters[4], (unsigned long long)sc.pairs);
-        atomicAdd(&counters[5], (unsigned long long)sc.tris);
+        add_counters(counters, cc, sc);
     }
 }
 
@@ -247,7 +252,7 @@ template <uint32_t LDS_DEPTH, bool COUNT, bool MULTI>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_refill(DevScene s0, const DevScene* __restrict__ scenes,
                                                              const WaveJob* __restrict__ jobs, PathState ps,
                                                              const uint32_t* __restrict__ closest_queue,
-                                                             const uint32_t* __restrict__ closest_count,
+                                                             const uint32_t* __restrict__ closest_count, uint32_t first,
                                                              const uint32_t* __restrict__ shadow_queue,
                                                              const uint32_t* __restrict__ shadow_count,
                                                              uint32_t* __restrict__ ovf,
@@ -276,7 +281,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_refill(DevScene s0, const
             const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
             opaque = s.all_opaque != 0u;
             rng.s = opaque ? 0u : ps.rng[p];
-            ray_begin(s, r, ld3(ps.ox, ps.oy, ps.oz, p), ld3(ps.dx, ps.dy, ps.dz, p), false, 0.0f, -1, st);
+            ray_begin(s, r, first ? camera_origin(s) : ld3(ps.ox, ps.oy, ps.oz, p), ld3(ps.dx, ps.dy, ps.dz, p), false,
+                      0.0f, -1, st);
         } else {
             p = shadow_queue[jj - nc];
             const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, MULTI ? ps.job[p] : 0u);
@@ -310,10 +316,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_refill(DevScene s0, const
         }
     }
     if (COUNT) {
-        atomicAdd(&counters[2], (unsigned long long)cc.pairs);
-        atomicAdd(&counters[3], (unsigned long long)cc.tris);
-        atomicAdd(&counters[4], (unsigned long long)sc.pairs);
-        atomicAdd(&counters[5], (unsigned long long)sc.tris);
+        add_counters(counters, cc, sc);
     }
 }
 
@@ -328,8 +331,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_refill(DevScene s0, const
 template <bool SHADOW, bool COUNT, bool MULTI>
 __device__ __forceinline__ void trace_phase(const DevScene& s0, const DevScene* __restrict__ scenes,
                                             const WaveJob* __restrict__ jobs, const PathState& ps,
-                                            const uint32_t* __restrict__ queue, uint32_t n, Stack& st,
-                                            TraversalCounters& ctr) {
+                                            const uint32_t* __restrict__ queue, uint32_t n, bool first,
+                                            Stack& st, TraversalCounters& ctr) {
     const uint32_t stride = gridDim.x * TRACE_BLOCK;
     uint32_t j = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     bool have = false, opaque = true;
@@ -350,7 +353,8 @@ __device__ __forceinline__ void trace_phase(const DevScene& s0, const DevScene* 
                 } else {
                     opaque = s.all_opaque != 0u;
                     rng.s = opaque ? 0u : ps.rng[p];
-                    ray_begin(s, r, ld3(ps.ox, ps.oy, ps.oz, p), ld3(ps.dx, ps.dy, ps.dz, p), false, 0.0f, -1, st);
+                    ray_begin(s, r, first ? camera_origin(s) : ld3(ps.ox, ps.oy, ps.oz, p), ld3(ps.dx, ps.dy, ps.dz, p),
+                              false, 0.0f, -1, st);
                 }
             }
             if (__ballot(have) == 0ull) break;
@@ -380,7 +384,7 @@ template <uint32_t LDS_DEPTH, bool COUNT, bool MULTI>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_fetch(DevScene s0, const DevScene* __restrict__ scenes,
                                                             const WaveJob* __restrict__ jobs, PathState ps,
                                                             const uint32_t* __restrict__ closest_queue,
-                                                            const uint32_t* __restrict__ closest_count,
+                                                            const uint32_t* __restrict__ closest_count, uint32_t first,
                                                             const uint32_t* __restrict__ shadow_queue,
                                                             const uint32_t* __restrict__ shadow_count,
                                                             uint32_t* __restrict__ ovf,
@@ -395,14 +399,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_fetch(DevScene s0, const 
     st.gtid = blockIdx.x * TRACE_BLOCK + threadIdx.x;
     TraversalCounters cc = {0, 0}, sc = {0, 0};
     if (closest_queue)
-        trace_phase<false, COUNT, MULTI>(s0, scenes, jobs, ps, closest_queue, *closest_count, st, cc);
+        trace_phase<false, COUNT, MULTI>(s0, scenes, jobs, ps, closest_queue, *closest_count, first != 0u, st, cc);
     if (shadow_queue)
-        trace_phase<true, COUNT, MULTI>(s0, scenes, jobs, ps, shadow_queue, *shadow_count, st, sc);
+        trace_phase<true, COUNT, MULTI>(s0, scenes, jobs, ps, shadow_queue, *shadow_count, false, st, sc);
     if (COUNT) {
-        atomicAdd(&counters[2], (unsigned long long)cc.pairs);
-        atomicAdd(&counters[3], (unsigned long long)cc.tris);
-        atomicAdd(&counters[4], (unsigned long long)sc.pairs);
-        atomicAdd(&counters[5], (unsigned long long)sc.tris);
+        add_counters(counters, cc, sc);
     }
 }
 
@@ -447,10 +448,12 @@ __global__ __launch_bounds__(WAVE_BLOCK, PT_SHADE_WAVES_PER_EU) void k_shade(Dev
             const uint32_t ji = MULTI ? ps.job[p] : 0u;
             const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, ji);
             const uint32_t max_bounces = job_of<MULTI>(j0, jobs, ji).max_bounces;
-            const f3 ray_origin = ld3(ps.ox, ps.oy, ps.oz, p), ray_direction = ld3(ps.dx, ps.dy, ps.dz, p);
-            f3 throughput = ld3(ps.tr, ps.tg, ps.tb, p);
-            f3 L = ld3(ps.lr, ps.lg, ps.lb, p);
-            const float previous_bsdf_pdf = ps.prev_pdf[p];
+            const bool first = bounce == 0u;  // camera ray: state implied (see k_generate)
+            const f3 ray_origin = first ? camera_origin(s) : ld3(ps.ox, ps.oy, ps.oz, p);
+            const f3 ray_direction = ld3(ps.dx, ps.dy, ps.dz, p);
+            f3 throughput = first ? f3{1.0f, 1.0f, 1.0f} : ld3(ps.tr, ps.tg, ps.tb, p);
+            f3 L = first ? f3{0.0f, 0.0f, 0.0f} : ld3(ps.lr, ps.lg, ps.lb, p);
+            const float previous_bsdf_pdf = first ? 0.0f : ps.prev_pdf[p];
             Rng rng{ps.rng[p]};
             const int32_t hit = ps.hit[p];
             if (hit == -1) {
@@ -641,8 +644,10 @@ struct EventScope {
 template <bool COUNT, bool MULTI>
 void launch_trace(uint32_t lds, uint32_t refill, uint32_t grid, hipStream_t st, const DevScene& s0,
                   const DevScene* scenes, const WaveJob* jobs, const PathState& ps, const uint32_t* cq,
-                  const uint32_t* cc, const uint32_t* sq, const uint32_t* scnt, uint32_t* ovf, unsigned long long* ctr) {
-#define PT_TRACE_LAUNCH(K, D) K<D, COUNT, MULTI><<<grid, TRACE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, cq, cc, sq, scnt, ovf, ctr)
+                  const uint32_t* cc, uint32_t first, const uint32_t* sq, const uint32_t* scnt, uint32_t* ovf,
+                  unsigned long long* ctr) {
+#define PT_TRACE_LAUNCH(K, D) \
+    K<D, COUNT, MULTI><<<grid, TRACE_BLOCK, 0, st>>>(s0, scenes, jobs, ps, cq, cc, first, sq, scnt, ovf, ctr)
     if (refill == TRAVERSAL_REFILL) {
         switch (lds) {
             case 24: PT_TRACE_LAUNCH(k_trace_refill, 24); break;
@@ -700,14 +705,15 @@ void run_chunk(Wavefront& w, const Launch& L, uint32_t max_bounces, bool countin
     uint32_t* nxt = w.queue[1];
     uint32_t* spare = w.queue[3];
     uint32_t* shadow = w.queue[2];
-    auto trace = [&](const uint32_t* cq, const uint32_t* cc, const uint32_t* sq, const uint32_t* sc) {
+    // first: the closest rays are camera rays (origin = camera position, not stored per path)
+    auto trace = [&](const uint32_t* cq, const uint32_t* cc, uint32_t first, const uint32_t* sq, const uint32_t* sc) {
         EventScope tm(lg, EV_EXTEND, stream);
         if (counting)
-            launch_trace<true, MULTI>(w.cfg.lds_depth, w.cfg.traversal, L.trace_grid, stream, *L.s0, L.scenes, L.jobs, ps, cq, cc, sq,
-                                      sc, w.ovf, ctr);
+            launch_trace<true, MULTI>(w.cfg.lds_depth, w.cfg.traversal, L.trace_grid, stream, *L.s0, L.scenes, L.jobs, ps,
+                                      cq, cc, first, sq, sc, w.ovf, ctr);
         else
-            launch_trace<false, MULTI>(w.cfg.lds_depth, w.cfg.traversal, L.trace_grid, stream, *L.s0, L.scenes, L.jobs, ps, cq, cc,
-                                       sq, sc, w.ovf, ctr);
+            launch_trace<false, MULTI>(w.cfg.lds_depth, w.cfg.traversal, L.trace_grid, stream, *L.s0, L.scenes, L.jobs, ps,
+                                       cq, cc, first, sq, sc, w.ovf, ctr);
     };
     for (uint32_t b = 0; b <= max_bounces; ++b) {
         uint32_t* cnt = w.counts + 2 * b;
@@ -715,7 +721,7 @@ void run_chunk(Wavefront& w, const Launch& L, uint32_t max_bounces, bool countin
             sort_queue<MULTI>(w, L, SORT_OCTANT, cur, cnt, spare);
             std::swap(cur, spare);
         }
-        trace(cur, cnt, b > 0 ? shadow : nullptr, b > 0 ? cnt - 1 : nullptr);
+        trace(cur, cnt, b == 0 ? 1u : 0u, b > 0 ? shadow : nullptr, b > 0 ? cnt - 1 : nullptr);
         if (w.cfg.sort_key == SORT_MATERIAL) {
             sort_queue<MULTI>(w, L, SORT_MATERIAL, cur, cnt, spare);
             std::swap(cur, spare);
@@ -731,7 +737,7 @@ void run_chunk(Wavefront& w, const Launch& L, uint32_t max_bounces, bool countin
         }
         std::swap(cur, nxt);
     }
-    trace(nullptr, nullptr, shadow, w.counts + 2 * max_bounces + 1);
+    trace(nullptr, nullptr, 0u, shadow, w.counts + 2 * max_bounces + 1);
     const uint32_t acc_grid = std::min<uint32_t>(grid, (L.units.n_units * 64u + WAVE_BLOCK - 1) / WAVE_BLOCK);
     k_accumulate<MULTI><<<acc_grid, WAVE_BLOCK, 0, stream>>>(*L.j0, L.jobs, L.units, ps);
 }

@@ -620,6 +620,9 @@ __device__ __forceinline__ bool ray_advance(const DevScene& s, RayState& r, Rng&
 }
 
 // ------------------------------------------------------------------ camera ray (trace.slang:736-748)
+__device__ __forceinline__ f3 camera_origin(const DevScene& s) {
+    return f3{s.camera.position[0], s.camera.position[1], s.camera.position[2]};
+}
 __device__ __forceinline__ f3 primary_direction(const DevScene& s, uint32_t px, uint32_t py, uint32_t iw, uint32_t ih,
                                                 Rng& rng) {
     float r0 = rng.next();
