@@ -1,0 +1,58 @@
+/*
+ * pt_assets.h - scene and sky files to the input of pt_world_build.
+ *
+ * Replaces the loading half of World::new (src/render/world.rs:589-611):
+ * Scene::new for glTF 2.0 scenes (world.rs:887-1423: images to RGBA8 texture
+ * atlases, node flattening with transforms, normals, MikkTSpace tangents,
+ * materials with OpenPBR extras) and the decode step of Skybox::new
+ * (world.rs:1872-1873). The arrays come out in the reference's input order,
+ * so pt_world_build then builds the BVH, alias tables and camera as the
+ * reference does. Supported inputs: .glb (or .gltf with embedded data URIs)
+ * with PNG images; sky images in Radiance .hdr or PNG.
+ */
+#ifndef PT_ASSETS_H
+#define PT_ASSETS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pt_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pt_assets pt_assets;
+
+typedef struct pt_assets_summary {
+    uint32_t triangle_count;
+    uint32_t material_count;
+    uint32_t srgb_atlas_count;   /* 8192^2 RGBA8 layers */
+    uint32_t linear_atlas_count;
+    uint32_t atlas_size;         /* 8192 when there are layers, else 0 */
+    uint32_t skybox_width;
+    uint32_t skybox_height;
+    uint32_t warning_count;      /* messages the reference would log (skipped primitives, extras) */
+} pt_assets_summary;
+
+pt_status pt_assets_create(pt_assets** out);
+pt_status pt_assets_destroy(pt_assets* assets);
+/* Scene::new on the bytes of a .glb / embedded .gltf; replaces a previous scene. */
+pt_status pt_assets_load_scene(pt_assets* assets, const uint8_t* bytes, size_t size);
+pt_status pt_assets_load_scene_file(pt_assets* assets, const char* path);
+/* image::load_from_memory(..).into_rgba32f() of a sky image; replaces a previous sky. */
+pt_status pt_assets_load_skybox(pt_assets* assets, const uint8_t* bytes, size_t size);
+pt_status pt_assets_load_skybox_file(pt_assets* assets, const char* path);
+pt_status pt_assets_info(const pt_assets* assets, pt_assets_summary* out);
+/* Warning `index` (< warning_count), or "" . */
+const char* pt_assets_warning(const pt_assets* assets, uint32_t index);
+/* A pt_world_desc borrowing the loaded arrays (valid until the next load or
+ * destroy): camera fields NaN (autogenerated), skybox_luminance -1 (the
+ * image's own), aspect ratio 1; the caller sets the render options. */
+pt_status pt_assets_world_desc(const pt_assets* assets, pt_world_desc* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PT_ASSETS_H */

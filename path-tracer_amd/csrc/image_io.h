@@ -1,0 +1,30 @@
+// image_io.h - image decoders of the asset front end (PNG textures, sky images).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace ptcore {
+
+// Decoded integer image: `channels` interleaved samples per pixel, 8 or 16
+// significant bits each (palettes expanded to RGB/RGBA, low-bit grey scaled to 8).
+struct DecodedImage {
+    uint32_t width = 0, height = 0;
+    int channels = 0, bits = 8;
+    std::vector<uint16_t> samples;
+};
+
+// RGBA32F image, rows top to bottom.
+struct FloatImage {
+    uint32_t width = 0, height = 0;
+    std::vector<float> rgba;
+};
+
+bool decode_png(const uint8_t* data, size_t size, DecodedImage& out, std::string& err);
+bool decode_hdr(const uint8_t* data, size_t size, FloatImage& out, std::string& err);
+// A sky image as image::load_from_memory(..).into_rgba32f() gives it (Radiance .hdr or PNG).
+bool decode_sky_image(const uint8_t* data, size_t size, FloatImage& out, std::string& err);
+
+}  // namespace ptcore

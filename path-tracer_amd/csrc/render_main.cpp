@@ -6,10 +6,11 @@
 //   --camera-pitch --camera-fov
 // and its tile loop (headless.rs:207-285): the image is traced tile by tile,
 // each tile to all of its samples, at most 16 samples per trace call, then
-// tone-mapped (display.wgsl) and written as RGB8. Scenes are the procedural
-// generators (cornell | field | large | sized:<triangles>) because .glb
-// ingest is not part of this core. Extra flags: --seed, --device,
-// --schedule mega|wavefront, --pfm <file> (linear storage image dump).
+// tone-mapped (display.wgsl) and written as RGB8. --scene takes a glTF file
+// (.glb, or .gltf with embedded data) with --skybox <.hdr|.png>, loaded by the
+// asset front end (pt_assets.h), or one of the procedural generators
+// (cornell | field | large | sized:<triangles>). Extra flags: --seed,
+// --device, --schedule mega|wavefront, --pfm <file> (linear storage image dump).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -17,6 +18,8 @@
 #include <cstring>
 #include <string>
 #include <vector>
+
+#include "../../include/pt_assets.h"
 
 #include "../../include/pt_api.h"
 #include "../../include/pt_scenes.h"
@@ -124,7 +127,7 @@ void write_pfm(const std::string& path, const std::vector<float>& rgb, uint32_t 
 }  // namespace
 
 int main(int argc, char** argv) {
-    std::string scene = "cornell", output = "render.png", pfm, schedule = "wavefront";
+    std::string scene = "cornell", skybox, output = "render.png", pfm, schedule = "wavefront";
     uint32_t width = 1920, height = 1080, samples = 256, max_bounces = PT_DEFAULT_MAX_BOUNCES, tile = 512, device = 0;
     uint64_t seed = 1;
     float sky_lum = -1.0f;
@@ -139,6 +142,7 @@ int main(int argc, char** argv) {
             return argv[++i];
         };
         if (a == "--scene") scene = val();
+        else if (a == "--skybox") skybox = val();
         else if (a == "--output") output = val();
         else if (a == "--pfm") pfm = val();
         else if (a == "--width") width = parse_u32(val(), "--width", 1, 1u << 16);
@@ -171,7 +175,8 @@ int main(int argc, char** argv) {
             fov = parse_finite(val(), "--camera-fov");
             if (!(fov > 0.0f && fov < 180.0f)) die("--camera-fov must be strictly between 0 and 180");
         } else if (a == "--help" || a == "-h") {
-            std::printf("usage: pt-render [render] --scene cornell|field|large|sized:N [--width W] [--height H]\n"
+            std::printf("usage: pt-render [render] --scene file.glb|cornell|field|large|sized:N [--skybox sky.hdr]\n"
+                        "  [--width W] [--height H]\n"
                         "  [--samples S] [--max-bounces D] [--tile-size T] [--output out.png] [--pfm out.pfm]\n"
                         "  [--skybox-luminance L] [--camera-position X,Y,Z] [--camera-yaw DEG]\n"
                         "  [--camera-pitch DEG] [--camera-fov DEG] [--seed N] [--schedule mega|wavefront]\n");
@@ -179,21 +184,39 @@ int main(int argc, char** argv) {
         } else die("unknown argument %s", a.c_str());
     }
     const auto started = std::chrono::steady_clock::now();
-    int kind = 0;
-    uint32_t tris = 0;
-    if (scene == "cornell") kind = PT_SCENE_CORNELL;
-    else if (scene == "field") kind = PT_SCENE_FIELD;
-    else if (scene == "large") kind = PT_SCENE_LARGE;
-    else if (scene.rfind("sized:", 0) == 0) {
-        kind = PT_SCENE_SIZED;
-        tris = parse_u32(scene.c_str() + 6, "--scene sized:N", 64, 1u << 30);
-    } else die("unknown scene %s (cornell, field, large, sized:N)", scene.c_str());
-
+    auto ends_with = [&](const char* suffix) {
+        const size_t n = std::strlen(suffix);
+        return scene.size() >= n && scene.compare(scene.size() - n, n, suffix) == 0;
+    };
     pt_generated* gen = nullptr;
-    check(pt_generate_scene(kind, seed, tris, width, height, &gen), "generate scene");
+    pt_assets* assets = nullptr;
     pt_world_desc wd;
-    check(pt_generated_world_desc(gen, &wd), "scene description");
-    if (sky_lum >= 0.0f) wd.skybox_luminance = sky_lum;
+    if (ends_with(".glb") || ends_with(".gltf")) {
+        if (skybox.empty()) die("--skybox <file> is required with a glTF scene");
+        check(pt_assets_create(&assets), "assets");
+        check(pt_assets_load_scene_file(assets, scene.c_str()), "load scene");
+        check(pt_assets_load_skybox_file(assets, skybox.c_str()), "load skybox");
+        pt_assets_summary info;
+        check(pt_assets_info(assets, &info), "assets");
+        for (uint32_t k = 0; k < info.warning_count; ++k) std::fprintf(stderr, "warning: %s\n", pt_assets_warning(assets, k));
+        check(pt_assets_world_desc(assets, &wd), "scene description");
+        // the headless default sky luminance (world.rs:21, headless.rs:24-31)
+        wd.skybox_luminance = sky_lum >= 0.0f ? sky_lum : 8000.0f;
+    } else {
+        if (!skybox.empty()) die("--skybox applies to glTF scenes; the generators make their own sky");
+        int kind = 0;
+        uint32_t tris = 0;
+        if (scene == "cornell") kind = PT_SCENE_CORNELL;
+        else if (scene == "field") kind = PT_SCENE_FIELD;
+        else if (scene == "large") kind = PT_SCENE_LARGE;
+        else if (scene.rfind("sized:", 0) == 0) {
+            kind = PT_SCENE_SIZED;
+            tris = parse_u32(scene.c_str() + 6, "--scene sized:N", 64, 1u << 30);
+        } else die("unknown scene %s (a .glb/.gltf file, cornell, field, large, sized:N)", scene.c_str());
+        check(pt_generate_scene(kind, seed, tris, width, height, &gen), "generate scene");
+        check(pt_generated_world_desc(gen, &wd), "scene description");
+        if (sky_lum >= 0.0f) wd.skybox_luminance = sky_lum;
+    }
     if (!std::isnan(cam_pos[0]))
         for (int k = 0; k < 3; ++k) wd.camera_position[k] = cam_pos[k];
     if (!std::isnan(yaw)) wd.camera_yaw_degrees = yaw;
@@ -213,6 +236,7 @@ int main(int argc, char** argv) {
     check(pt_scene_create(ctx, &sd, &dscene), "upload scene");
     pt_world_destroy(world);  // the GPU has its own copy (headless.rs:157)
     pt_generated_destroy(gen);
+    pt_assets_destroy(assets);
 
     const uint32_t tw = std::min(tile, width), th = std::min(tile, height);
     pt_tracer* tracer = nullptr;

@@ -170,6 +170,12 @@ class WorldUpdates(C.Structure):
                 ("reset_accumulation", C.c_uint32), ("selected_material", C.c_int32)]
 
 
+class AssetsSummary(C.Structure):
+    _fields_ = [("triangle_count", C.c_uint32), ("material_count", C.c_uint32), ("srgb_atlas_count", C.c_uint32),
+                ("linear_atlas_count", C.c_uint32), ("atlas_size", C.c_uint32), ("skybox_width", C.c_uint32),
+                ("skybox_height", C.c_uint32), ("warning_count", C.c_uint32)]
+
+
 SORT_NONE, SORT_OCTANT, SORT_MATERIAL = 0, 1, 2
 TRAVERSAL_BATCH, TRAVERSAL_REFILL, TRAVERSAL_FETCH = 0, 1, 2
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
@@ -232,6 +238,15 @@ _SIGNATURES = [
     ("pt_server_result", _i32, [_p, C.c_uint64, _p]),
     ("pt_server_cancel", _i32, [_p, C.c_uint64]),
     ("pt_server_stats_get", _i32, [_p, C.POINTER(ServerStats)]),
+    ("pt_assets_create", _i32, [C.POINTER(_p)]),
+    ("pt_assets_destroy", _i32, [_p]),
+    ("pt_assets_load_scene", _i32, [_p, _p, C.c_size_t]),
+    ("pt_assets_load_scene_file", _i32, [_p, C.c_char_p]),
+    ("pt_assets_load_skybox", _i32, [_p, _p, C.c_size_t]),
+    ("pt_assets_load_skybox_file", _i32, [_p, C.c_char_p]),
+    ("pt_assets_info", _i32, [_p, C.POINTER(AssetsSummary)]),
+    ("pt_assets_warning", C.c_char_p, [_p, _u32]),
+    ("pt_assets_world_desc", _i32, [_p, C.POINTER(WorldDesc)]),
     ("pt_generate_scene", _i32, [C.c_int, C.c_uint64, _u32, _u32, _u32, C.POINTER(_p)]),
     ("pt_generated_world_desc", _i32, [_p, C.POINTER(WorldDesc)]),
     ("pt_generated_destroy", _i32, [_p]),
@@ -452,6 +467,84 @@ def triangles_from_vertices(verts, material_ids):
     t4 = np.concatenate([t, np.ones((n, 1))], axis=1).astype(np.float32)
     attr["t0"], attr["t1"], attr["t2"] = t4, t4, t4
     return geo, attr
+
+
+class Assets:
+    """Scene and sky files (pt_assets.h): Scene::new for .glb and the sky image decode."""
+
+    def __init__(self, scene=None, skybox=None):
+        h = C.c_void_p()
+        _check(lib().pt_assets_create(C.byref(h)))
+        self.handle = h
+        if scene is not None:
+            self.load_scene(scene)
+        if skybox is not None:
+            self.load_skybox(skybox)
+
+    def _load(self, fn_bytes, fn_file, src):
+        if isinstance(src, (bytes, bytearray, memoryview)):
+            buf = (C.c_uint8 * len(src)).from_buffer_copy(bytes(src))
+            _check(fn_bytes(self.handle, buf, len(src)))
+        else:
+            _check(fn_file(self.handle, os.fsencode(str(src))))
+
+    def load_scene(self, src):
+        """src: bytes of a .glb / embedded .gltf, or a path."""
+        self._load(lib().pt_assets_load_scene, lib().pt_assets_load_scene_file, src)
+
+    def load_skybox(self, src):
+        """src: bytes of a Radiance .hdr / PNG, or a path."""
+        self._load(lib().pt_assets_load_skybox, lib().pt_assets_load_skybox_file, src)
+
+    def info(self):
+        s = AssetsSummary()
+        _check(lib().pt_assets_info(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in AssetsSummary._fields_}
+
+    def warnings(self):
+        return [lib().pt_assets_warning(self.handle, i).decode() for i in range(self.info()["warning_count"])]
+
+    def world_desc(self, skybox_luminance=-1.0, camera_position=None, yaw=None, pitch=None, fov=None, aspect=1.0):
+        d = WorldDesc()
+        _check(lib().pt_assets_world_desc(self.handle, C.byref(d)))
+        d.skybox_luminance = skybox_luminance
+        if camera_position is not None:
+            for k in range(3):
+                d.camera_position[k] = camera_position[k]
+        if yaw is not None:
+            d.camera_yaw_degrees = yaw
+        if pitch is not None:
+            d.camera_pitch_degrees = pitch
+        if fov is not None:
+            d.camera_fov_degrees = fov
+        d.aspect_ratio = aspect
+        return d
+
+    def world(self, **options):
+        """pt_world_build over the loaded arrays (World::new)."""
+        return World(self.world_desc(**options), keep=self)
+
+    def arrays(self):
+        d = self.world_desc()
+        layer = d.atlas_size * d.atlas_size * 4
+        return {"geometries": _view(d.geometries, GEOMETRY_DTYPE, d.triangle_count),
+                "attributes": _view(d.attributes, ATTRIBUTE_DTYPE, d.triangle_count),
+                "materials": _view(d.materials, MATERIAL_DTYPE, d.material_count),
+                "srgb_atlases": _view(d.srgb_atlases, np.uint8, layer * d.srgb_atlas_count),
+                "linear_atlases": _view(d.linear_atlases, np.uint8, layer * d.linear_atlas_count),
+                "skybox": _view(d.skybox_rgba, np.float32, d.skybox_width * d.skybox_height * 4).reshape(
+                    d.skybox_height, d.skybox_width, 4)}
+
+    def close(self):
+        if self.handle:
+            lib().pt_assets_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class WorldInput:

@@ -8,7 +8,7 @@ import numpy as np
 import ptcore
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("pt_api.h", "pt_scenes.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("pt_api.h", "pt_scenes.h", "pt_assets.h")]
 
 
 def declared_functions():
