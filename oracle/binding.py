@@ -169,3 +169,25 @@ def occluded(desc, rays, tmax, skip=None):
 
 def f32vec(*xs):
     return np.ascontiguousarray(np.array(xs, np.float32))
+
+
+# ---- the kernel source compiled for the host (kernel_on_host.hip)
+_koh = None
+
+
+def kernel_on_host(desc, image_w, image_h, max_bounces, sample, tile, origin=(0, 0)):
+    """One sample (1-based index `sample`) of a tile by the device code of
+    pt_device.h run on the CPU; returns the per-pixel path radiance (h, w, 3)."""
+    global _koh
+    if _koh is None:
+        path = os.path.join(_HERE, "libkernel_on_host.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} not built (make -C oracle)")
+        _koh = C.CDLL(path)
+        _koh.kernel_on_host_render.restype = C.c_int
+        _koh.kernel_on_host_render.argtypes = [C.POINTER(ptcore.SceneDesc)] + [C.c_uint32] * 8 + [_p]
+    tw, th = tile
+    out = np.zeros((th, tw, 3), np.float32)
+    assert _koh.kernel_on_host_render(C.byref(desc), image_w, image_h, origin[0], origin[1], tw, th, max_bounces,
+                                      sample, _ptr(out)) == 0
+    return out

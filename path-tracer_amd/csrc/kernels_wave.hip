@@ -464,10 +464,11 @@ __global__ __launch_bounds__(WAVE_BLOCK, PT_SHADE_WAVES_PER_EU) void k_shade(Dev
             } else {
                 SurfaceHit sf;
                 Bsdf b;
-                f3 emitted;
+                f3 emission;
+                float emission_mis;
                 setup_surface(s, (uint32_t)hit, ps.ht[p], ps.hv[p], ps.hw[p], ray_origin, ray_direction, bounce,
-                              previous_bsdf_pdf, rng, sf, b, emitted);
-                L = L + throughput * emitted;
+                              previous_bsdf_pdf, rng, sf, b, emission, emission_mis);
+                L = L + throughput * emission * emission_mis;
                 const f3 next_origin = sf.hit_point + sf.flat_N * fmaxf(1e-4f, 1e-4f * ptm::length(sf.hit_point));
                 LightSample light = sample_light(s, next_origin, rng);
                 const f3 lwi = {ptm::dot(sf.Tt, light.direction), ptm::dot(sf.Bt, light.direction), ptm::dot(sf.N, light.direction)};

@@ -753,15 +753,17 @@ __device__ __forceinline__ f3 estimate_e_dielectric(f3 wo, float opl, float ax, 
 }
 
 // Shading frame of a hit (trace.slang:929-1091) and BSDF setup (1097-1181).
-// Consumes the estimate_e_dielectric draws. Returns the emission (already
-// MIS-weighted) in `emitted`.
+// Consumes the estimate_e_dielectric draws. Returns the emission and its MIS
+// weight separately: the caller adds (throughput * emission) * weight, the
+// shader's left-to-right order (trace.slang:991).
 struct SurfaceHit {
     f3 hit_point, flat_N, Tt, Bt, N, wo;
 };
 
 __device__ __forceinline__ void setup_surface(const DevScene& s, uint32_t hit, float hit_t, float hit_v, float hit_w,
                                               f3 ray_origin, f3 ray_direction, uint32_t bounce,
-                                              float previous_bsdf_pdf, Rng& rng, SurfaceHit& sf, Bsdf& b, f3& emitted) {
+                                              float previous_bsdf_pdf, Rng& rng, SurfaceHit& sf, Bsdf& b, f3& emission,
+                                              float& emission_mis) {
     const PtTriangleAttribute& at = s.attr[hit];
     const float hit_u = 1.0f - hit_v - hit_w;
     float iu, iv;
@@ -793,14 +795,13 @@ __device__ __forceinline__ void setup_surface(const DevScene& s, uint32_t hit, f
         float4 t = tex_uv(s.srgb_lut, s.linear, s.atlas_size, mat.specular_weight_tex_layer, mat.specular_weight_uv, iu, iv, false);
         b.specular_weight *= t.w;
     }
-    f3 emission = evaluate_emission(s, mat, iu, iv);
-    float emission_mis = 1.0f;
+    emission = evaluate_emission(s, mat, iu, iv);
+    emission_mis = 1.0f;
     const float prob_geo = s.uniforms.light_probabilities[0];
     if (bounce > 0u && prob_geo > 0.0f && ptm::any_pos(emission)) {
         float light_pdf = triangle_light_pdf(s, hit, ray_direction, hit_t * hit_t, prob_geo * triangle_selection_probability(s, hit));
         emission_mis = power_heuristic(previous_bsdf_pdf, light_pdf);
     }
-    emitted = emission * emission_mis;
 
     sf.hit_point = ray_direction * hit_t + ray_origin;
     const f3 V = -ray_direction;
@@ -898,6 +899,71 @@ __device__ __forceinline__ f3 sample_bsdf(const Bsdf& b, f3 wo, Rng& rng) {
         wi = ptm::normalize(wu.x * bx + wu.y * by + wu.z * bz);
     }
     return wi;
+}
+
+// ------------------------------------------------------------------ whole path (the megakernel's per-thread loop)
+// One path sample (trace.slang:726-1297), returning final_color.
+template <bool COUNT>
+__device__ __forceinline__ f3 trace_path(const DevScene& s, uint32_t px, uint32_t py, uint32_t iw, uint32_t ih,
+                                         uint32_t max_bounces, uint32_t sample_count, Stack& st,
+                                         TraversalCounters& ctr, uint32_t& closest_rays, uint32_t& shadow_rays) {
+    Rng rng{ptm::rng_seed(px, py, sample_count)};
+    f3 ray_origin = {s.camera.position[0], s.camera.position[1], s.camera.position[2]};
+    f3 ray_direction = primary_direction(s, px, py, iw, ih, rng);
+    f3 rcp = f3{1.0f, 1.0f, 1.0f} / ray_direction;
+    f3 throughput = {1.0f, 1.0f, 1.0f};
+    f3 final_color = {0.0f, 0.0f, 0.0f};
+    float previous_bsdf_pdf = 0.0f;
+    const float prob_sky = s.uniforms.light_probabilities[1];
+    for (uint32_t bounce = 0; bounce <= max_bounces; ++bounce) {
+        float hit_t, hit_v, hit_w;
+        if (COUNT) ++closest_rays;
+        int32_t hit = closest_hit<COUNT>(s, ray_origin, ray_direction, rcp, rng, st, hit_t, hit_v, hit_w, ctr);
+        if (hit == -1) {
+            float mis = 1.0f;
+            if (bounce > 0u) mis = power_heuristic(previous_bsdf_pdf, skybox_light_pdf(s, ray_direction, prob_sky));
+            final_color = final_color + throughput * evaluate_skybox(s, ray_direction) * mis;
+            break;
+        }
+        SurfaceHit sf;
+        Bsdf b;
+        f3 emission;
+        float emission_mis;
+        setup_surface(s, (uint32_t)hit, hit_t, hit_v, hit_w, ray_origin, ray_direction, bounce, previous_bsdf_pdf, rng,
+                      sf, b, emission, emission_mis);
+        final_color = final_color + throughput * emission * emission_mis;
+        f3 next_origin = sf.hit_point + sf.flat_N * fmaxf(1e-4f, 1e-4f * ptm::length(sf.hit_point));
+        LightSample light = sample_light(s, next_origin, rng);
+        f3 lwi = {ptm::dot(sf.Tt, light.direction), ptm::dot(sf.Bt, light.direction), ptm::dot(sf.N, light.direction)};
+        if (light.pdf > 0.0f && lwi.z > 0.0f && ptm::any_pos(light.emission)) {
+            f3 f;
+            float pdf;
+            evaluate_lobes(b, sf.wo, lwi, f, pdf);
+            if (COUNT) ++shadow_rays;
+            if (!occluded<COUNT>(s, next_origin, light.direction, light.distance, light.triangle_index, rng, st, ctr)) {
+                float weight = bounce < max_bounces ? power_heuristic(light.pdf, pdf) : 1.0f;
+                final_color = final_color + throughput * f * lwi.z * light.emission * (weight / light.pdf);
+            }
+        }
+        if (bounce == max_bounces) break;
+        f3 wi = sample_bsdf(b, sf.wo, rng);
+        if (wi.z <= 0.0f) break;
+        f3 f;
+        float pdf;
+        evaluate_lobes(b, sf.wo, wi, f, pdf);
+        if (pdf <= 0.0f) break;
+        throughput = throughput * (f * wi.z / pdf);
+        previous_bsdf_pdf = pdf;
+        if (bounce > 3u) {
+            float p_survive = ptm::clamp_(fmaxf(throughput.x, fmaxf(throughput.y, throughput.z)), 0.05f, 0.95f);
+            if (rng.next() > p_survive) break;
+            throughput = throughput / p_survive;
+        }
+        ray_origin = next_origin;
+        ray_direction = wi.x * sf.Tt + wi.y * sf.Bt + wi.z * sf.N;
+        rcp = f3{1.0f, 1.0f, 1.0f} / ray_direction;
+    }
+    return final_color;
 }
 
 }  // namespace ptd
