@@ -95,6 +95,7 @@ def test_batch_matches_oracle(ctx, scenes):
     ref = oracle.render_f32(worlds[si].desc, w, h, mb, spp)[0]
     g, r = t.accumulation()[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
     assert np.abs(g - r).sum() / np.abs(r).sum() <= 1e-4
+    assert np.array_equal(t.accumulation()[..., :3], ref[..., :3])  # and bit-exact
     t.close()
     other.close()
 
