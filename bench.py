@@ -259,6 +259,9 @@ def main():
                          "(default: library default)")
     ap.add_argument("--sort", type=int, default=None, choices=[0, 1, 2], help="queue sort key")
     ap.add_argument("--lds", type=int, default=None, choices=[16, 24, 32], help="LDS stack depth")
+    ap.add_argument("--chunk", type=int, default=None, help="paths per wavefront chunk (default: library default)")
+    ap.add_argument("--streams", type=int, default=None, choices=[0, 1, 2],
+                    help="wavefront chunk lanes on separate streams (default: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -293,7 +296,8 @@ def main():
     ctx.synchronize()
     upload_s = time.perf_counter() - t0
     schedule = ptcore.SCHEDULE_WAVEFRONT if args.schedule == "wavefront" else ptcore.SCHEDULE_MEGAKERNEL
-    knobs = {k: v for k, v in (("traversal", args.traversal), ("sort_key", args.sort), ("lds_stack_depth", args.lds))
+    knobs = {k: v for k, v in (("traversal", args.traversal), ("sort_key", args.sort), ("lds_stack_depth", args.lds),
+                               ("streams", args.streams), ("chunk_paths", args.chunk))
              if v is not None}
     if knobs:
         ptcore.set_wave_config(ctx, **{**ptcore.get_wave_config(ctx), **knobs})

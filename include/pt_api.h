@@ -296,6 +296,9 @@ typedef struct pt_wave_config {
                                   1: lane refill (a lane that finishes starts its next ray at once);
                                   2: threshold refill (idle lanes start their next rays together
                                      once fewer than half of the wave is still traversing) */
+    uint32_t streams;          /* 1, or 2: chunks alternate between two HIP streams with their own
+                                  path state, so one chunk's shading overlaps the other's traversal;
+                                  0: automatic (2 when a render spans three chunks or more) */
 } pt_wave_config;
 #define PT_TRAVERSAL_BATCH 0
 #define PT_TRAVERSAL_REFILL 1

@@ -36,6 +36,7 @@ pt_status check_config(const pt_wave_config* c) {
         return set_error(PT_ERR_INVALID, "chunk_paths must be in [4096, 2^27]");
     if (c->sort_key > PT_SORT_MATERIAL) return set_error(PT_ERR_INVALID, "unknown sort key %u", c->sort_key);
     if (c->traversal > 2) return set_error(PT_ERR_INVALID, "unknown traversal mode %u", c->traversal);
+    if (c->streams > 2) return set_error(PT_ERR_INVALID, "streams must be 0 (automatic), 1 or 2");
     return PT_OK;
 }
 
@@ -82,6 +83,7 @@ pt_status pt_context_set_wave_config(pt_context* c, const pt_wave_config* cfg) {
     c->wave_cfg.chunk_paths = cfg->chunk_paths;
     c->wave_cfg.sort_key = cfg->sort_key;
     c->wave_cfg.traversal = cfg->traversal;
+    c->wave_cfg.streams = cfg->streams;
     return PT_OK;
 }
 
@@ -91,6 +93,7 @@ pt_status pt_context_get_wave_config(const pt_context* c, pt_wave_config* cfg) {
     cfg->chunk_paths = c->wave_cfg.chunk_paths;
     cfg->sort_key = c->wave_cfg.sort_key;
     cfg->traversal = c->wave_cfg.traversal;
+    cfg->streams = c->wave_cfg.streams;
     return PT_OK;
 }
 
@@ -127,16 +130,22 @@ pt_status pt_autotune_batch(pt_tracer* const* tracers, uint32_t count, const uin
     const uint32_t chunk_opts[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
     const uint32_t sort_opts[] = {ptd::SORT_NONE, ptd::SORT_OCTANT, ptd::SORT_MATERIAL};
     const uint32_t traversal_opts[] = {ptd::TRAVERSAL_BATCH, ptd::TRAVERSAL_REFILL, ptd::TRAVERSAL_FETCH};
-    for (int knob = 0; knob < 4; ++knob) {
-        const uint32_t* opts = knob == 0 ? traversal_opts : knob == 1 ? chunk_opts : (knob == 2 ? sort_opts : lds_opts);
-        const int nopts = knob == 0 ? 3 : (knob == 1 ? 4 : 3);
+    const uint32_t stream_opts[] = {0, 1, 2};
+    for (int knob = 0; knob < 5; ++knob) {
+        const uint32_t* opts = knob == 0   ? traversal_opts
+                               : knob == 1 ? chunk_opts
+                               : knob == 2 ? sort_opts
+                               : knob == 3 ? lds_opts
+                                           : stream_opts;
+        const int nopts = knob == 1 ? 4 : 3;
         ptd::WaveConfig base = best;
         for (int k = 0; k < nopts; ++k) {
             ptd::WaveConfig cand = base;
             if (knob == 0) cand.traversal = opts[k];
             else if (knob == 1) cand.chunk_paths = opts[k];
             else if (knob == 2) cand.sort_key = opts[k];
-            else cand.lds_depth = opts[k];
+            else if (knob == 3) cand.lds_depth = opts[k];
+            else cand.streams = opts[k];
             if (std::memcmp(&cand, &base, sizeof(cand)) == 0) continue;
             if ((st = trial(cand, ms))) return st;
             ++trials;
@@ -152,6 +161,7 @@ pt_status pt_autotune_batch(pt_tracer* const* tracers, uint32_t count, const uin
     out->best.chunk_paths = best.chunk_paths;
     out->best.sort_key = best.sort_key;
     out->best.traversal = best.traversal;
+    out->best.streams = best.streams;
     out->best_ms = best_ms;
     out->trials = trials;
     return PT_OK;

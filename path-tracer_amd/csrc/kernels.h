@@ -31,6 +31,7 @@ struct WaveConfig {
     uint32_t chunk_paths = 1u << 24;  // paths in flight per chunk (queue tile size)
     uint32_t sort_key = SORT_NONE;    // ray-queue ordering
     uint32_t traversal = 0;           // a PT_TRAVERSAL_* mode
+    uint32_t streams = 0;             // chunk lanes running concurrently: 1, 2, or 0 = automatic
 };
 enum Traversal : uint32_t { TRAVERSAL_BATCH = 0, TRAVERSAL_REFILL = 1, TRAVERSAL_FETCH = 2 };
 

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <new>
 
 namespace ptd {
 
@@ -779,10 +780,33 @@ EventLog::~EventLog() {
 }
 
 Wavefront::~Wavefront() {
+    if (partner) {
+        hipStreamSynchronize(partner->own_stream);
+        delete partner;
+    }
     if (arena) hipFree(arena);
     if (ovf) hipFree(ovf);
     if (tables) hipFree(tables);
     if (staging) hipHostFree(staging);
+    if (ev_start) hipEventDestroy(ev_start);
+    if (ev_done) hipEventDestroy(ev_done);
+    if (own_stream) hipStreamDestroy(own_stream);
+}
+
+pt_status Wavefront::ensure_overflow(size_t need, hipStream_t stream) {
+    if (need <= ovf_bytes) return PT_OK;
+    if (ovf) {
+        hipStreamSynchronize(stream);
+        hipFree(ovf);
+        ovf = nullptr;
+    }
+    hipError_t e = hipMalloc(&ovf, need);
+    if (e != hipSuccess) {
+        ovf_bytes = 0;
+        return hip_error(e, "wavefront overflow stacks");
+    }
+    ovf_bytes = need;
+    return PT_OK;
 }
 
 pt_status Wavefront::reserve(uint32_t paths) {
@@ -846,7 +870,18 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
         if (h.block_begin > be || be > tile_blocks(j)) return set_error(PT_ERR_INVALID, "job block range out of the tile");
         total_units += be - h.block_begin;
     }
-    const uint32_t chunk_cap = std::max(cfg.chunk_paths, max_unit_paths);
+    uint64_t total_paths = 0;
+    for (const HostJob& h : jobs) {
+        const uint32_t be = h.block_end ? h.block_end : tile_blocks(h.job);
+        total_paths += (uint64_t)(be - h.block_begin) * 64u * h.job.samples;
+    }
+    uint32_t chunk_cap = std::max(cfg.chunk_paths, max_unit_paths);
+    // lanes: 2 on request (then at least two chunks, so they overlap), or
+    // automatically when the job spans three chunks or more: a chunk's last,
+    // small bounces then run beside the next chunk's first, large ones
+    const uint32_t lanes = cfg.streams == 2 || (cfg.streams == 0 && total_paths > 2ull * chunk_cap) ? 2u : 1u;
+    if (cfg.streams == 2 && total_paths <= chunk_cap)
+        chunk_cap = std::max<uint32_t>(max_unit_paths, (uint32_t)((total_paths + 1) / 2));
     pt_status st = reserve(chunk_cap);
     if (st) return st;
     const uint32_t grid = ctx->cu_count * 8u;
@@ -855,18 +890,22 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
     const uint32_t lds_min = std::min(cfg.lds_depth, MEGA_LDS_DEPTH);
     const size_t need = (size_t)(max_depth > lds_min ? max_depth - lds_min : 1u) *
                         std::max<size_t>((size_t)shade_grid * WAVE_BLOCK, (size_t)trace_grid * TRACE_BLOCK) * 4u;
-    if (need > ovf_bytes) {
-        if (ovf) {
-            hipStreamSynchronize(stream);
-            hipFree(ovf);
-            ovf = nullptr;
+    if ((st = ensure_overflow(need, stream))) return st;
+    Wavefront* lane[2] = {this, nullptr};
+    if (lanes == 2) {
+        if (!partner) {
+            partner = new (std::nothrow) Wavefront();
+            if (!partner) return set_error(PT_ERR_MEMORY, "out of host memory");
+            partner->ctx = ctx;
+            hipError_t e = hipStreamCreateWithFlags(&partner->own_stream, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_start, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_done, hipEventDisableTiming);
+            if (e != hipSuccess) return hip_error(e, "second wavefront lane");
+            partner->stream_override = partner->own_stream;
         }
-        hipError_t e = hipMalloc(&ovf, need);
-        if (e != hipSuccess) {
-            ovf_bytes = 0;
-            return hip_error(e, "wavefront overflow stacks");
-        }
-        ovf_bytes = need;
+        partner->cfg = cfg;
+        if ((st = partner->reserve(chunk_cap)) || (st = partner->ensure_overflow(need, partner->own_stream))) return st;
+        lane[1] = partner;
     }
     // host tables: scenes, jobs, then the units of every chunk (job, block, offset)
     std::vector<DevScene> scene_tab;
@@ -958,12 +997,19 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
     unsigned long long* ctr = counting ? ctx->counters : nullptr;
     if (counting) hipMemsetAsync(ctx->counters, 0, 16 * sizeof(unsigned long long), stream);
     EventScope total(lg, EV_TOTAL, stream);
+    const bool two = lanes == 2 && chunks.size() > 1;
+    if (two) {  // the second lane starts after the tables are on the device
+        hipEventRecord(ev_start, stream);
+        hipStreamWaitEvent(partner->own_stream, ev_start, 0);
+    }
     for (size_t c = 0; c < chunks.size(); ++c) {
+        Wavefront& w = *lane[two ? c % 2 : 0];
+        const hipStream_t lane_stream = &w == this ? stream : w.own_stream;
         Launch L;
         L.grid = grid;
         L.trace_grid = trace_grid;
         L.shade_grid = shade_grid;
-        L.stream = stream;
+        L.stream = lane_stream;
         L.s0 = &scene_tab[0];
         L.j0 = &job_tab[0];
         L.scenes = d_scenes;
@@ -974,19 +1020,23 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
         L.units.n_units = (uint32_t)(chunks[c].unit_end - chunks[c].unit_begin);
         L.units.n_paths = chunks[c].paths;
         L.any_alpha = any_alpha;
-        if (multi) run_chunk<true>(*this, L, max_bounces, counting, lg, ctr);
-        else run_chunk<false>(*this, L, max_bounces, counting, lg, ctr);
+        if (multi) run_chunk<true>(w, L, max_bounces, counting, lg, ctr);
+        else run_chunk<false>(w, L, max_bounces, counting, lg, ctr);
         e = hipGetLastError();
         if (e != hipSuccess) return hip_error(e, "wavefront launch");
         if (counting) {
             uint32_t h[MAX_COUNTS];
-            hipMemcpyAsync(h, counts, sizeof(h), hipMemcpyDeviceToHost, stream);
-            hipStreamSynchronize(stream);
+            hipMemcpyAsync(h, w.counts, sizeof(h), hipMemcpyDeviceToHost, lane_stream);
+            hipStreamSynchronize(lane_stream);
             for (uint32_t b = 0; b <= max_bounces; ++b) {
                 host_counters[0] += h[2 * b];
                 host_counters[1] += h[2 * b + 1];
             }
         }
+    }
+    if (two) {  // the caller's stream resumes after both lanes
+        hipEventRecord(ev_done, partner->own_stream);
+        hipStreamWaitEvent(stream, ev_done, 0);
     }
     return PT_OK;
 }

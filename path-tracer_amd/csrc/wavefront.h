@@ -105,6 +105,12 @@ struct Wavefront {
     size_t staging_bytes = 0;
     ptd::WaveConfig cfg;
     hipStream_t stream_override = nullptr;  // run on this stream instead of the context's
+    // second chunk lane (cfg.streams == 2): its own path state, overflow
+    // stacks and stream; odd chunks run there, overlapping the even ones
+    Wavefront* partner = nullptr;
+    hipStream_t own_stream = nullptr;       // created for a partner lane
+    hipEvent_t ev_start = nullptr, ev_done = nullptr;
+    pt_status ensure_overflow(size_t bytes, hipStream_t stream);
     pt_status reserve(uint32_t paths);
     // Trace every job completely. profiling: 0 off, 1 per-kernel event
     // timing into `log`, 2 timing + traversal counters (synchronises per chunk).

@@ -130,7 +130,7 @@ class TraceStats(C.Structure):
 
 class WaveConfig(C.Structure):
     _fields_ = [("lds_stack_depth", C.c_uint32), ("chunk_paths", C.c_uint32), ("sort_key", C.c_uint32),
-                ("traversal", C.c_uint32)]
+                ("traversal", C.c_uint32), ("streams", C.c_uint32)]
 
 
 class TuneResult(C.Structure):
@@ -791,13 +791,13 @@ class Tracer:
             pass
 
 
-DEFAULT_WAVE_CONFIG = dict(lds_stack_depth=16, chunk_paths=1 << 24, sort_key=SORT_NONE, traversal=0)
+DEFAULT_WAVE_CONFIG = dict(lds_stack_depth=16, chunk_paths=1 << 24, sort_key=SORT_NONE, traversal=0, streams=0)
 
 
 def set_wave_config(ctx: Context, **overrides):
     """Set the context's wavefront tuning; unspecified knobs take the defaults."""
     v = {**DEFAULT_WAVE_CONFIG, **overrides}
-    cfg = WaveConfig(v["lds_stack_depth"], v["chunk_paths"], v["sort_key"], v["traversal"])
+    cfg = WaveConfig(v["lds_stack_depth"], v["chunk_paths"], v["sort_key"], v["traversal"], v["streams"])
     _check(lib().pt_context_set_wave_config(ctx.handle, C.byref(cfg)))
 
 

@@ -63,7 +63,8 @@ def test_batch_equals_individual_renders(ctx, scenes):
                                  dict(lds_stack_depth=24), dict(lds_stack_depth=32), dict(chunk_paths=1 << 16),
                                  dict(traversal=1), dict(traversal=1, sort_key=ptcore.SORT_OCTANT, lds_stack_depth=24),
                                  dict(traversal=2), dict(traversal=2, sort_key=ptcore.SORT_MATERIAL, lds_stack_depth=32,
-                                                         chunk_paths=1 << 16)])
+                                                         chunk_paths=1 << 16),
+                                 dict(streams=2), dict(streams=2, chunk_paths=1 << 16, sort_key=ptcore.SORT_OCTANT)])
 def test_wave_configs_are_bit_identical(ctx, scenes, cfg):
     _, sc = scenes
     base = render_alone(ctx, sc, JOBS[:3])
