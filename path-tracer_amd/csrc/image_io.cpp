@@ -1,5 +1,6 @@
 // image_io.cpp - decoders for the image files the reference's front end reads:
-// PNG (glTF textures and sky images) and Radiance RGBE .hdr (sky images).
+// PNG (glTF textures and sky images) and Radiance RGBE .hdr (sky images);
+// OpenEXR skies are in exr_decode.cpp.
 //
 // The reference decodes through the `image` crate (Cargo.toml: image 0.25,
 // via gltf::import_slice for textures and image::load_from_memory for the
@@ -251,8 +252,8 @@ bool decode_sky_image(const uint8_t* data, size_t size, FloatImage& out, std::st
         return true;
     }
     if (size >= 4 && data[0] == 0x76 && data[1] == 0x2f && data[2] == 0x31 && data[3] == 0x01)
-        return fail(err, "OpenEXR sky images are not supported by this build (convert to Radiance .hdr)");
-    return fail(err, "unrecognised sky image format (supported: Radiance .hdr, PNG)");
+        return decode_exr(data, size, out, err);
+    return fail(err, "unrecognised sky image format (supported: Radiance .hdr, OpenEXR, PNG)");
 }
 
 }  // namespace ptcore

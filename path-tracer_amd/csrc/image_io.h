@@ -24,7 +24,9 @@ struct FloatImage {
 
 bool decode_png(const uint8_t* data, size_t size, DecodedImage& out, std::string& err);
 bool decode_hdr(const uint8_t* data, size_t size, FloatImage& out, std::string& err);
-// A sky image as image::load_from_memory(..).into_rgba32f() gives it (Radiance .hdr or PNG).
+// Single-part scanline OpenEXR (NONE, ZIPS, ZIP, PIZ; HALF/FLOAT/UINT channels) to RGBA32F.
+bool decode_exr(const uint8_t* data, size_t size, FloatImage& out, std::string& err);
+// A sky image as image::load_from_memory(..).into_rgba32f() gives it (Radiance .hdr, OpenEXR or PNG).
 bool decode_sky_image(const uint8_t* data, size_t size, FloatImage& out, std::string& err);
 
 }  // namespace ptcore

@@ -1,5 +1,5 @@
 """Writers for small test inputs of the asset front end: glTF binaries (.glb),
-PNG images and Radiance .hdr skies, from numpy arrays (test helpers only)."""
+PNG images, Radiance .hdr and OpenEXR skies, from numpy arrays (test helpers only)."""
 import json
 import struct
 import zlib
@@ -139,3 +139,45 @@ def rgbe_to_float(rgbe):
     e = rgbe[..., 3].astype(np.int32)
     scale = np.where(e > 0, np.ldexp(np.float32(1.0), e - 136).astype(np.float32), np.float32(0.0))
     return rgbe[..., :3].astype(np.float32) * scale[..., None]
+
+
+def exr(channels, compression=0, lines_per_chunk=None):
+    """Scanline OpenEXR bytes. channels: {name: HxW array} of float16 (HALF) or
+    float32 (FLOAT); compression 0 NONE, 2 ZIPS, 3 ZIP."""
+    names = sorted(channels)
+    h, w = channels[names[0]].shape
+    lpc = lines_per_chunk or {0: 1, 2: 1, 3: 16}[compression]
+
+    def attr(name, typ, data):
+        return name.encode() + b"\0" + typ.encode() + b"\0" + struct.pack("<I", len(data)) + data
+
+    chl = b"".join(n.encode() + b"\0" + struct.pack("<iBBBBii", 1 if channels[n].dtype == np.float16 else 2,
+                                                       0, 0, 0, 0, 1, 1) for n in names) + b"\0"
+    head = struct.pack("<II", 20000630, 2)
+    head += attr("channels", "chlist", chl)
+    head += attr("compression", "compression", bytes([compression]))
+    head += attr("dataWindow", "box2i", struct.pack("<iiii", 0, 0, w - 1, h - 1))
+    head += attr("displayWindow", "box2i", struct.pack("<iiii", 0, 0, w - 1, h - 1))
+    head += attr("lineOrder", "lineOrder", b"\0")
+    head += attr("pixelAspectRatio", "float", struct.pack("<f", 1.0))
+    head += attr("screenWindowCenter", "v2f", struct.pack("<ff", 0, 0))
+    head += attr("screenWindowWidth", "float", struct.pack("<f", 1.0)) + b"\0"
+    chunks = []
+    for y0 in range(0, h, lpc):
+        raw = b"".join(channels[n][y].astype("<f2" if channels[n].dtype == np.float16 else "<f4").tobytes()
+                       for y in range(y0, min(h, y0 + lpc)) for n in names)
+        data = raw
+        if compression in (2, 3):
+            b = np.frombuffer(raw, np.uint8)
+            t = np.concatenate([b[0::2], b[1::2]]).astype(np.int32)
+            d = t.copy()
+            d[1:] = (t[1:] - t[:-1] + 128) & 255
+            z = zlib.compress(d.astype(np.uint8).tobytes())
+            data = z if len(z) < len(raw) else raw
+        chunks.append(struct.pack("<iI", y0, len(data)) + data)
+    table_at = len(head)
+    offs, o = [], table_at + 8 * len(chunks)
+    for c in chunks:
+        offs.append(o)
+        o += len(c)
+    return head + struct.pack(f"<{len(offs)}Q", *offs) + b"".join(chunks)

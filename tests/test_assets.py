@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 import ptcore
-from glb_util import GlbBuilder, hdr, png, rgbe_to_float
+from glb_util import GlbBuilder, exr, hdr, png, rgbe_to_float
 
 REF = "/root/reference/assets"
 SKY = hdr(np.array([[[128, 128, 128, 129], [64, 32, 16, 130]]], np.uint8), rle=False)
@@ -336,3 +336,35 @@ def test_reference_default_assets_load():
     w = a.world(skybox_luminance=8000.0, aspect=16 / 9)
     assert w.desc.triangle_count == info["triangle_count"]
     assert w.desc.skybox.average_luminance == 8000.0
+
+
+@pytest.mark.parametrize("compression", [0, 2, 3])
+def test_exr_sky_channels_and_compression(compression):
+    rng = np.random.default_rng(compression + 1)
+    h, w = 37, 23
+    r = rng.uniform(0, 40, (h, w)).astype(np.float32)
+    g = rng.uniform(0, 1, (h, w)).astype(np.float16)
+    b = np.round(rng.uniform(0, 3, (h, w))).astype(np.float32)  # runs compress under ZIP
+    a = ptcore.Assets(quad_glb(), exr({"R": r, "G": g, "B": b}, compression))
+    sky = a.arrays()["skybox"]
+    assert sky.shape == (h, w, 4)
+    assert np.array_equal(sky[..., 0], r) and np.array_equal(sky[..., 1], g.astype(np.float32))
+    assert np.array_equal(sky[..., 2], b) and np.all(sky[..., 3] == 1.0)  # no A channel: opaque
+    a2 = ptcore.Assets(quad_glb(), exr({"R": r, "G": r, "B": r, "A": np.zeros((h, w), np.float32)}, compression))
+    assert np.all(a2.arrays()["skybox"][..., 3] == 0.0)
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="the reference's asset files are not present")
+def test_reference_4k_exr_sky_matches_its_2k_hdr():
+    # the reference's 4k sky is PIZ-compressed FLOAT RGBA; its 2k .hdr is the same sky, RGBE-quantised:
+    # the 4k image averaged over 2x2 pixels must agree with it to the RGBE quantisation level
+    a = ptcore.Assets(quad_glb(), os.path.join(REF, "skybox", "kloppenheim_06_puresky_4k.exr"))
+    ex = a.arrays()["skybox"].copy()
+    b = ptcore.Assets(quad_glb(), os.path.join(REF, "skybox", "kloppenheim_06_puresky_2k.hdr"))
+    hd = b.arrays()["skybox"]
+    assert ex.shape == (2048, 4096, 4) and np.isfinite(ex).all() and np.all(ex[..., 3] == 1.0)
+    lum = lambda x: (x[..., :3].astype(np.float64) * [0.2126, 0.7152, 0.0722]).sum(-1)
+    le, lh = lum(ex.reshape(1024, 2, 2048, 2, 4).mean((1, 3))), lum(hd)
+    rel = np.abs(le - lh) / np.maximum(lh, 1e-6)
+    assert np.median(rel) < 0.01 and np.percentile(rel, 99) < 0.03
+    assert np.corrcoef(le.ravel(), lh.ravel())[0, 1] > 0.9999
