@@ -9,6 +9,7 @@
  * so pt_world_build then builds the BVH, alias tables and camera as the
  * reference does. Supported inputs: .glb (or .gltf with embedded data URIs)
  * with PNG images; sky images in Radiance .hdr, scanline OpenEXR (NONE, ZIPS, ZIP, PIZ) or PNG.
+ * Also the output-image writers of the `render` command (PNG, JPEG, WebP).
  */
 #ifndef PT_ASSETS_H
 #define PT_ASSETS_H
@@ -50,6 +51,28 @@ const char* pt_assets_warning(const pt_assets* assets, uint32_t index);
  * destroy): camera fields NaN (autogenerated), skybox_luminance -1 (the
  * image's own), aspect ratio 1; the caller sets the render options. */
 pt_status pt_assets_world_desc(const pt_assets* assets, pt_world_desc* out);
+
+/* Output images: the saving step of `render` (src/render/headless.rs:118-127,
+ * 163-170), image::save_buffer(path, rgb8, width, height, Rgb8) with the
+ * format taken from the file extension. PNG and WebP are lossless; JPEG is
+ * baseline with 4:4:4 sampling at quality 75 unless asked otherwise. */
+typedef enum pt_image_format {
+    PT_IMAGE_PNG = 0,
+    PT_IMAGE_JPEG = 1,
+    PT_IMAGE_WEBP = 2, /* lossless (VP8L) */
+} pt_image_format;
+
+/* image::ImageFormat::from_path restricted to the formats `render` accepts
+ * (parse_output, headless.rs:118-127): .png, .jpg, .jpeg, .webp, any case. */
+pt_status pt_image_format_from_path(const char* path, pt_image_format* format);
+/* Encodes width x height RGB8 pixels (rows row_stride bytes apart, 0 = width * 3).
+ * quality: JPEG 1..100, 0 = 75; ignored by the lossless formats. *out is
+ * allocated by the library and released with pt_image_free. */
+pt_status pt_image_encode(pt_image_format format, const uint8_t* rgb, size_t row_stride, uint32_t width,
+                          uint32_t height, uint32_t quality, uint8_t** out, size_t* size);
+void pt_image_free(uint8_t* data);
+/* image::save_buffer: encode by extension and write the file. */
+pt_status pt_image_save(const char* path, const uint8_t* rgb, uint32_t width, uint32_t height);
 
 #ifdef __cplusplus
 }

@@ -6,8 +6,9 @@
 //   --camera-pitch --camera-fov
 // and its tile loop (headless.rs:207-285): the image is traced tile by tile,
 // each tile to all of its samples, at most 16 samples per trace call, then
-// tone-mapped (display.wgsl) and written as RGB8. --scene takes a glTF file
-// (.glb, or .gltf with embedded data) with --skybox <.hdr|.png>, loaded by the
+// tone-mapped (display.wgsl) and saved as RGB8 in the format the --output
+// extension names (.png, .jpg/.jpeg, .webp). --scene takes a glTF file
+// (.glb, or .gltf with embedded data) with --skybox <.hdr|.exr|.png>, loaded by the
 // asset front end (pt_assets.h), or one of the procedural generators
 // (cornell | field | large | sized:<triangles>). Extra flags: --seed,
 // --device, --schedule mega|wavefront, --pfm <file> (linear storage image dump).
@@ -51,71 +52,6 @@ uint32_t parse_u32(const char* v, const char* flag, uint32_t lo, uint32_t hi) {
     return (uint32_t)x;
 }
 
-uint32_t crc32(const uint8_t* p, size_t n, uint32_t c = 0xffffffffu) {
-    static uint32_t table[256];
-    static bool init = false;
-    if (!init) {
-        for (uint32_t i = 0; i < 256; ++i) {
-            uint32_t k = i;
-            for (int j = 0; j < 8; ++j) k = (k & 1) ? 0xedb88320u ^ (k >> 1) : k >> 1;
-            table[i] = k;
-        }
-        init = true;
-    }
-    for (size_t i = 0; i < n; ++i) c = table[(c ^ p[i]) & 255u] ^ (c >> 8);
-    return c;
-}
-
-// PNG with zlib "stored" (uncompressed) deflate blocks: RGB8, no filtering
-void write_png(const std::string& path, const std::vector<uint8_t>& rgb, uint32_t w, uint32_t h) {
-    std::vector<uint8_t> raw;
-    raw.reserve((size_t)(w * 3 + 1) * h);
-    for (uint32_t y = 0; y < h; ++y) {
-        raw.push_back(0);
-        raw.insert(raw.end(), rgb.begin() + (size_t)y * w * 3, rgb.begin() + (size_t)(y + 1) * w * 3);
-    }
-    std::vector<uint8_t> z = {0x78, 0x01};
-    uint32_t a = 1, b = 0;
-    for (uint8_t c : raw) {
-        a = (a + c) % 65521u;
-        b = (b + a) % 65521u;
-    }
-    for (size_t off = 0; off < raw.size() || off == 0; off += 65535) {
-        size_t len = std::min<size_t>(65535, raw.size() - off);
-        z.push_back(off + len >= raw.size() ? 1 : 0);
-        z.push_back(len & 255);
-        z.push_back(len >> 8);
-        z.push_back(~len & 255);
-        z.push_back((~len >> 8) & 255);
-        z.insert(z.end(), raw.begin() + off, raw.begin() + off + len);
-        if (len == 0) break;
-    }
-    uint32_t adler = (b << 16) | a;
-    for (int k = 3; k >= 0; --k) z.push_back((adler >> (8 * k)) & 255);
-    FILE* f = std::fopen(path.c_str(), "wb");
-    if (!f) die("cannot write %s", path.c_str());
-    const uint8_t sig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
-    std::fwrite(sig, 1, 8, f);
-    auto chunk = [&](const char* type, const std::vector<uint8_t>& data) {
-        uint8_t len[4] = {(uint8_t)(data.size() >> 24), (uint8_t)(data.size() >> 16), (uint8_t)(data.size() >> 8),
-                          (uint8_t)data.size()};
-        std::fwrite(len, 1, 4, f);
-        std::vector<uint8_t> td(type, type + 4);
-        td.insert(td.end(), data.begin(), data.end());
-        std::fwrite(td.data(), 1, td.size(), f);
-        uint32_t c = crc32(td.data(), td.size()) ^ 0xffffffffu;
-        uint8_t cb[4] = {(uint8_t)(c >> 24), (uint8_t)(c >> 16), (uint8_t)(c >> 8), (uint8_t)c};
-        std::fwrite(cb, 1, 4, f);
-    };
-    std::vector<uint8_t> ihdr = {(uint8_t)(w >> 24), (uint8_t)(w >> 16), (uint8_t)(w >> 8), (uint8_t)w,
-                                 (uint8_t)(h >> 24), (uint8_t)(h >> 16), (uint8_t)(h >> 8), (uint8_t)h,
-                                 8, 2, 0, 0, 0};
-    chunk("IHDR", ihdr);
-    chunk("IDAT", z);
-    chunk("IEND", {});
-    std::fclose(f);
-}
-
 void write_pfm(const std::string& path, const std::vector<float>& rgb, uint32_t w, uint32_t h) {
     FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) die("cannot write %s", path.c_str());
@@ -143,7 +79,11 @@ int main(int argc, char** argv) {
         };
         if (a == "--scene") scene = val();
         else if (a == "--skybox") skybox = val();
-        else if (a == "--output") output = val();
+        else if (a == "--output") {
+            output = val();
+            pt_image_format f;  // rejected here, before any work, as parse_output does (headless.rs:118-127)
+            check(pt_image_format_from_path(output.c_str(), &f), "--output");
+        }
         else if (a == "--pfm") pfm = val();
         else if (a == "--width") width = parse_u32(val(), "--width", 1, 1u << 16);
         else if (a == "--height") height = parse_u32(val(), "--height", 1, 1u << 16);
@@ -177,7 +117,7 @@ int main(int argc, char** argv) {
         } else if (a == "--help" || a == "-h") {
             std::printf("usage: pt-render [render] --scene file.glb|cornell|field|large|sized:N [--skybox sky.hdr]\n"
                         "  [--width W] [--height H]\n"
-                        "  [--samples S] [--max-bounces D] [--tile-size T] [--output out.png] [--pfm out.pfm]\n"
+                        "  [--samples S] [--max-bounces D] [--tile-size T] [--output out.png|.jpg|.webp] [--pfm out.pfm]\n"
                         "  [--skybox-luminance L] [--camera-position X,Y,Z] [--camera-yaw DEG]\n"
                         "  [--camera-pitch DEG] [--camera-fov DEG] [--seed N] [--schedule mega|wavefront]\n");
             return 0;
@@ -269,7 +209,7 @@ int main(int argc, char** argv) {
         }
     }
     const double trace_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - trace_start).count();
-    write_png(output, rgb, width, height);
+    check(pt_image_save(output.c_str(), rgb.data(), width, height), "save");
     if (!pfm.empty()) write_pfm(pfm, lin, width, height);
     const double total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - started).count();
     std::printf("Saved %s in %.1fs (trace %.3fs, %.3g path samples/s)\n", output.c_str(), total_s, trace_s,

@@ -247,6 +247,11 @@ _SIGNATURES = [
     ("pt_assets_info", _i32, [_p, C.POINTER(AssetsSummary)]),
     ("pt_assets_warning", C.c_char_p, [_p, _u32]),
     ("pt_assets_world_desc", _i32, [_p, C.POINTER(WorldDesc)]),
+    ("pt_image_format_from_path", _i32, [C.c_char_p, C.POINTER(C.c_int)]),
+    ("pt_image_encode", _i32, [C.c_int, _p, C.c_size_t, _u32, _u32, _u32, C.POINTER(C.POINTER(C.c_uint8)),
+                               C.POINTER(C.c_size_t)]),
+    ("pt_image_free", None, [C.POINTER(C.c_uint8)]),
+    ("pt_image_save", _i32, [C.c_char_p, _p, _u32, _u32]),
     ("pt_generate_scene", _i32, [C.c_int, C.c_uint64, _u32, _u32, _u32, C.POINTER(_p)]),
     ("pt_generated_world_desc", _i32, [_p, C.POINTER(WorldDesc)]),
     ("pt_generated_destroy", _i32, [_p]),
@@ -469,6 +474,41 @@ def triangles_from_vertices(verts, material_ids):
     return geo, attr
 
 
+IMAGE_FORMATS = {"png": 0, "jpeg": 1, "webp": 2}
+
+
+def image_format_from_path(path):
+    """'png', 'jpeg' or 'webp' by extension (render's parse_output, headless.rs:118-127)."""
+    f = C.c_int()
+    _check(lib().pt_image_format_from_path(os.fsencode(str(path)), C.byref(f)))
+    return {v: k for k, v in IMAGE_FORMATS.items()}[f.value]
+
+
+def _rgb8(rgb):
+    rgb = np.asarray(rgb)
+    if rgb.dtype != np.uint8 or rgb.ndim != 3 or rgb.shape[2] != 3:
+        raise ValueError("expected an HxWx3 uint8 array")
+    return rgb if rgb.strides[1:] == (3, 1) else np.ascontiguousarray(rgb)
+
+
+def encode_image(rgb, fmt="png", quality=0):
+    """Bytes of an HxWx3 uint8 image as PNG, JPEG (quality 1..100, 0 = 75) or lossless WebP."""
+    rgb = _rgb8(rgb)
+    out, size = C.POINTER(C.c_uint8)(), C.c_size_t()
+    _check(lib().pt_image_encode(IMAGE_FORMATS[fmt], _ptr(rgb), rgb.strides[0], rgb.shape[1], rgb.shape[0],
+                                 quality, C.byref(out), C.byref(size)))
+    try:
+        return C.string_at(out, size.value)
+    finally:
+        lib().pt_image_free(out)
+
+
+def save_image(path, rgb):
+    """image::save_buffer of RGB8 pixels, format by extension (headless.rs:163-170)."""
+    rgb = np.ascontiguousarray(_rgb8(rgb))
+    _check(lib().pt_image_save(os.fsencode(str(path)), _ptr(rgb), rgb.shape[1], rgb.shape[0]))
+
+
 class Assets:
     """Scene and sky files (pt_assets.h): Scene::new for .glb and the sky image decode."""
 
@@ -493,7 +533,7 @@ class Assets:
         self._load(lib().pt_assets_load_scene, lib().pt_assets_load_scene_file, src)
 
     def load_skybox(self, src):
-        """src: bytes of a Radiance .hdr / PNG, or a path."""
+        """src: bytes of a Radiance .hdr / OpenEXR / PNG, or a path."""
         self._load(lib().pt_assets_load_skybox, lib().pt_assets_load_skybox_file, src)
 
     def info(self):

@@ -1,6 +1,7 @@
 """GPU: scenes loaded by the asset front end (a .glb with PNG textures, an
 emitter and a Radiance .hdr sky) render on the HIP path exactly as the CPU
 oracle renders the same ingested arrays, and pt-render writes the image."""
+import io
 import math
 import os
 import subprocess
@@ -12,6 +13,11 @@ import binding as oracle
 import ptcore
 from glb_util import GlbBuilder, hdr, png
 from test_gpu_parity import assert_parity
+
+try:
+    from PIL import Image
+except ImportError:  # the lossless/lossy comparison needs an independent decoder
+    Image = None
 
 pytestmark = pytest.mark.gpu
 
@@ -105,11 +111,21 @@ def test_pt_render_writes_the_image(tmp_path):
     glb, sky = scene_files()
     (tmp_path / "scene.glb").write_bytes(glb)
     (tmp_path / "sky.hdr").write_bytes(sky)
-    out = tmp_path / "render.png"
-    r = subprocess.run([os.path.join(ROOT, "path-tracer_amd", "pt-render"), "render", "--scene", str(tmp_path / "scene.glb"),
-                        "--skybox", str(tmp_path / "sky.hdr"), "--width", "96", "--height", "64", "--samples", "4",
-                        "--tile-size", "48", "--output", str(out)], capture_output=True, text=True, timeout=120)
-    print(r.stdout, r.stderr)
-    assert r.returncode == 0
-    data = out.read_bytes()
-    assert data[:8] == b"\x89PNG\r\n\x1a\n" and len(data) > 96 * 64
+    decoded = {}
+    for ext in ["png", "webp", "jpg"]:
+        out = tmp_path / f"render.{ext}"
+        r = subprocess.run([os.path.join(ROOT, "path-tracer_amd", "pt-render"), "render", "--scene",
+                            str(tmp_path / "scene.glb"), "--skybox", str(tmp_path / "sky.hdr"), "--width", "96",
+                            "--height", "64", "--samples", "4", "--tile-size", "48", "--output", str(out)],
+                           capture_output=True, text=True, timeout=120)
+        print(ext, r.stdout, r.stderr)
+        assert r.returncode == 0
+        data = out.read_bytes()
+        assert data[:4] in (b"\x89PNG", b"RIFF", b"\xff\xd8\xff\xe0")
+        if Image is not None:
+            decoded[ext] = np.asarray(Image.open(io.BytesIO(data)).convert("RGB")).astype(np.int32)
+    if decoded:
+        # the same deterministic render saved three ways: lossless files agree exactly, JPEG closely
+        assert decoded["png"].shape == (64, 96, 3)
+        assert np.array_equal(decoded["png"], decoded["webp"])
+        assert np.abs(decoded["jpg"] - decoded["png"]).mean() < 4.0
