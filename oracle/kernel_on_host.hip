@@ -76,7 +76,7 @@ extern "C" int kernel_on_host_render(const pt_scene_desc* d, uint32_t image_w, u
     s.all_opaque = all_opaque ? 1u : 0u;
     std::vector<uint32_t> lds(4096);
     Stack st;
-    st.lds = lds.data();
+    st.lds = (lds_u32*)lds.data();  // plain memory in this host build
     st.lds_stride = 1;
     st.lds_depth = (uint32_t)lds.size();
     st.ovf = nullptr;

@@ -23,7 +23,7 @@ __global__ __launch_bounds__(256) void k_megakernel(DevScene s, TileArgs a, floa
     __shared__ uint32_t lds_stack[MEGA_LDS_DEPTH * 256];
     const uint32_t tid = threadIdx.x;
     Stack st;
-    st.lds = lds_stack + tid;
+    st.lds = as_lds(lds_stack + tid);
     st.lds_stride = 256;
     st.lds_depth = MEGA_LDS_DEPTH;
     st.ovf = ovf;
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void k_primary_hits(DevScene s, TileArgs a, in
     __shared__ uint32_t lds_stack[MEGA_LDS_DEPTH * 256];
     const uint32_t tid = threadIdx.x;
     Stack st;
-    st.lds = lds_stack + tid;
+    st.lds = as_lds(lds_stack + tid);
     st.lds_stride = 256;
     st.lds_depth = MEGA_LDS_DEPTH;
     st.ovf = ovf;
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void k_intersect(DevScene s, const float* __re
     __shared__ uint32_t lds_stack[MEGA_LDS_DEPTH * 256];
     const uint32_t tid = threadIdx.x;
     Stack st;
-    st.lds = lds_stack + tid;
+    st.lds = as_lds(lds_stack + tid);
     st.lds_stride = 256;
     st.lds_depth = MEGA_LDS_DEPTH;
     st.ovf = ovf;
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_occluded(DevScene s, const float* __res
     __shared__ uint32_t lds_stack[MEGA_LDS_DEPTH * 256];
     const uint32_t tid = threadIdx.x;
     Stack st;
-    st.lds = lds_stack + tid;
+    st.lds = as_lds(lds_stack + tid);
     st.lds_stride = 256;
     st.lds_depth = MEGA_LDS_DEPTH;
     st.ovf = ovf;

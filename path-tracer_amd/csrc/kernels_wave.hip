@@ -205,7 +205,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene s0, const DevSce
                                                       uint32_t* __restrict__ ovf, unsigned long long* __restrict__ counters) {
     __shared__ uint32_t lds_stack[LDS_DEPTH * TRACE_BLOCK];
     Stack st;
-    st.lds = lds_stack + threadIdx.x;
+    st.lds = as_lds(lds_stack + threadIdx.x);
     st.lds_stride = TRACE_BLOCK;
     st.lds_depth = LDS_DEPTH;
     st.ovf = ovf;
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_refill(DevScene s0, const
                                                              unsigned long long* __restrict__ counters) {
     __shared__ uint32_t lds_stack[LDS_DEPTH * TRACE_BLOCK];
     Stack st;
-    st.lds = lds_stack + threadIdx.x;
+    st.lds = as_lds(lds_stack + threadIdx.x);
     st.lds_stride = TRACE_BLOCK;
     st.lds_depth = LDS_DEPTH;
     st.ovf = ovf;
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_fetch(DevScene s0, const 
                                                             unsigned long long* __restrict__ counters) {
     __shared__ uint32_t lds_stack[LDS_DEPTH * TRACE_BLOCK];
     Stack st;
-    st.lds = lds_stack + threadIdx.x;
+    st.lds = as_lds(lds_stack + threadIdx.x);
     st.lds_stride = TRACE_BLOCK;
     st.lds_depth = LDS_DEPTH;
     st.ovf = ovf;
@@ -427,7 +427,7 @@ __global__ __launch_bounds__(WAVE_BLOCK, PT_SHADE_WAVES_PER_EU) void k_shade(Dev
                                                       uint32_t* __restrict__ ovf) {
     __shared__ uint32_t lds_stack[INLINE_SHADOW ? MEGA_LDS_DEPTH * WAVE_BLOCK : 1];
     Stack st;
-    st.lds = lds_stack + (INLINE_SHADOW ? threadIdx.x : 0u);
+    st.lds = as_lds(lds_stack + (INLINE_SHADOW ? threadIdx.x : 0u));
     st.lds_stride = WAVE_BLOCK;
     st.lds_depth = MEGA_LDS_DEPTH;
     st.ovf = ovf;

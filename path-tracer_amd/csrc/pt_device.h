@@ -319,9 +319,18 @@ __host__ __device__ __forceinline__ uint32_t child_code(uint32_t node_index, uin
 }
 
 // Per-thread traversal stack: `lds_depth` entries in LDS (lane-interleaved,
-// conflict-free), the rest in a grid-strided global overflow area.
+// conflict-free), the rest in a grid-strided global overflow area. In the
+// device build the LDS part is typed as LDS (address space 3), so its reads
+// and writes are ds_read/ds_write and never merge with the overflow accesses
+// into flat (either-space) instructions.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+#else
+typedef uint32_t lds_u32;
+#endif
+__device__ __forceinline__ lds_u32* as_lds(uint32_t* p) { return (lds_u32*)p; }
 struct Stack {
-    uint32_t* lds;        // lds[k * stride + lane]
+    lds_u32* lds;         // lds[k * stride + lane]
     uint32_t* ovf;        // ovf[(k - lds_depth) * ovf_stride + gtid]
     uint32_t lds_stride;
     uint32_t lds_depth;
@@ -333,10 +342,14 @@ struct Stack {
         else ovf[(size_t)(sp - lds_depth) * ovf_stride + gtid] = e;
         ++sp;
     }
+    // The LDS read is unconditional (clamped index) and the overflow read is a
+    // separate, rarely taken branch: a pop reads LDS with a ds_read and waits
+    // only for LDS, instead of one flat load through a selected address space.
     __device__ __forceinline__ uint32_t pop() {
         --sp;
-        if (sp < lds_depth) return lds[sp * lds_stride];
-        return ovf[(size_t)(sp - lds_depth) * ovf_stride + gtid];
+        uint32_t e = lds[(sp < lds_depth ? sp : lds_depth - 1u) * lds_stride];
+        if (sp >= lds_depth) e = ovf[(size_t)(sp - lds_depth) * ovf_stride + gtid];
+        return e;
     }
 };
 
