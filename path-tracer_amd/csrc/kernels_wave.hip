@@ -38,11 +38,16 @@ constexpr uint32_t WAVE_BLOCK = 256;
 #ifndef PT_TRACE_WAVES_PER_CU
 #define PT_TRACE_WAVES_PER_CU 256u
 #endif
+// rounds of WAVE_BLOCK queue entries that a shade block partitions into hits and misses together
+#ifndef PT_SHADE_WINDOW
+#define PT_SHADE_WINDOW 2u
+#endif
 // blocks per CU of the shade launches
 #ifndef PT_SHADE_BLOCKS_PER_CU
 #define PT_SHADE_BLOCKS_PER_CU 32u
 #endif
 constexpr uint32_t TRACE_BLOCK = PT_TRACE_BLOCK;
+constexpr uint32_t SHADE_WINDOW = PT_SHADE_WINDOW;
 constexpr uint32_t MAX_COUNTS = 2 * (PT_MAX_BOUNCES_LIMIT + 2);
 constexpr uint32_t SORT_BUCKETS = 256;
 
@@ -438,92 +443,140 @@ __global__ __launch_bounds__(WAVE_BLOCK, PT_SHADE_WAVES_PER_EU) void k_shade(Dev
     BlockQueue nq{nq_buf, &nq_n, &nq_base}, sq{sq_buf, &sq_n, &sq_base};
     bq_init(nq);
     bq_init(sq);
+    // The block takes SHADE_WINDOW rounds of WAVE_BLOCK queue entries at a time
+    // and reorders them in LDS: paths whose ray hit a surface first, then the
+    // missed ones, each in queue order. A wave then runs either the surface
+    // work (BSDF setup, E_dielectric, NEE, sampling: thousands of
+    // instructions) or the sky lookup, instead of idling its missed lanes
+    // through the surface work. Each path's computation is unchanged.
+    constexpr uint32_t WIN = SHADE_WINDOW, NW = WAVE_BLOCK / 64u;
+    __shared__ uint32_t win[WIN * WAVE_BLOCK];
+    __shared__ uint32_t win_hits[WIN * NW], win_misses[WIN * NW];
+    const uint32_t wave = threadIdx.x >> 6;
     const uint32_t count = *count_ptr;
-    const uint32_t stride = gridDim.x * WAVE_BLOCK;
-    for (uint32_t base = blockIdx.x * WAVE_BLOCK; base < count; base += stride) {
-        const uint32_t j = base + threadIdx.x;
-        bool cont = false, shadow = false;
-        uint32_t p = 0;
-        if (j < count) {
-            p = queue[j];
-            const uint32_t ji = MULTI ? ps.job[p] : 0u;
-            const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, ji);
-            const uint32_t max_bounces = job_of<MULTI>(j0, jobs, ji).max_bounces;
-            const bool first = bounce == 0u;  // camera ray: state implied (see k_generate)
-            const f3 ray_origin = first ? camera_origin(s) : ld3(ps.ox, ps.oy, ps.oz, p);
-            const f3 ray_direction = ld3(ps.dx, ps.dy, ps.dz, p);
-            f3 throughput = first ? f3{1.0f, 1.0f, 1.0f} : ld3(ps.tr, ps.tg, ps.tb, p);
-            f3 L = first ? f3{0.0f, 0.0f, 0.0f} : ld3(ps.lr, ps.lg, ps.lb, p);
-            const float previous_bsdf_pdf = first ? 0.0f : ps.prev_pdf[p];
-            Rng rng{ps.rng[p]};
-            const int32_t hit = ps.hit[p];
-            if (hit == -1) {
-                float mis = 1.0f;
-                if (bounce > 0u)
-                    mis = power_heuristic(previous_bsdf_pdf, skybox_light_pdf(s, ray_direction, s.uniforms.light_probabilities[1]));
-                L = L + throughput * evaluate_skybox(s, ray_direction) * mis;
-            } else {
-                SurfaceHit sf;
-                Bsdf b;
-                f3 emission;
-                float emission_mis;
-                setup_surface(s, (uint32_t)hit, ps.ht[p], ps.hv[p], ps.hw[p], ray_origin, ray_direction, bounce,
-                              previous_bsdf_pdf, rng, sf, b, emission, emission_mis);
-                L = L + throughput * emission * emission_mis;
-                const f3 next_origin = sf.hit_point + sf.flat_N * fmaxf(1e-4f, 1e-4f * ptm::length(sf.hit_point));
-                LightSample light = sample_light(s, next_origin, rng);
-                const f3 lwi = {ptm::dot(sf.Tt, light.direction), ptm::dot(sf.Bt, light.direction), ptm::dot(sf.N, light.direction)};
-                if (light.pdf > 0.0f && lwi.z > 0.0f && ptm::any_pos(light.emission)) {
-                    f3 f;
-                    float pdf;
-                    evaluate_lobes(b, sf.wo, lwi, f, pdf);
-                    const float weight = bounce < max_bounces ? power_heuristic(light.pdf, pdf) : 1.0f;
-                    const f3 contrib = throughput * f * lwi.z * light.emission * (weight / light.pdf);
-                    if (INLINE_SHADOW && !s.all_opaque) {
-                        TraversalCounters ctr = {0, 0};
-                        if (!occluded<false>(s, next_origin, light.direction, light.distance, light.triangle_index, rng, st, ctr))
-                            L = L + contrib;
-                    } else {
-                        shadow = true;
-                        st3(ps.sox, ps.soy, ps.soz, p, next_origin);
-                        st3(ps.sdx, ps.sdy, ps.sdz, p, light.direction);
-                        ps.sdist[p] = light.distance;
-                        ps.slight[p] = light.triangle_index;
-                        st3(ps.scr, ps.scg, ps.scb, p, contrib);
-                    }
-                }
-                if (bounce < max_bounces) {
-                    const f3 wi = sample_bsdf(b, sf.wo, rng);
-                    if (wi.z > 0.0f) {
+    const uint32_t stride = gridDim.x * WAVE_BLOCK * WIN;
+    for (uint32_t wbase = blockIdx.x * WAVE_BLOCK * WIN; wbase < count; wbase += stride) {
+        uint32_t pk[WIN];
+        bool hk[WIN], mk[WIN];
+#pragma unroll
+        for (uint32_t k = 0; k < WIN; ++k) {
+            const uint32_t j = wbase + k * WAVE_BLOCK + threadIdx.x;
+            const bool valid = j < count;
+            pk[k] = valid ? queue[j] : 0u;
+            const int32_t h = valid ? ps.hit[pk[k]] : 0;
+            hk[k] = valid && h != -1;
+            mk[k] = valid && h == -1;
+            const uint32_t nh = (uint32_t)__popcll(__ballot(hk[k])), nm = (uint32_t)__popcll(__ballot(mk[k]));
+            if (lane_id() == 0u) {
+                win_hits[k * NW + wave] = nh;
+                win_misses[k * NW + wave] = nm;
+            }
+        }
+        __syncthreads();
+        uint32_t off_h[WIN], off_m[WIN], run_h = 0, run_m = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < WIN * NW; ++i) {
+            if (i % NW == wave) {
+                off_h[i / NW] = run_h;
+                off_m[i / NW] = run_m;
+            }
+            run_h += win_hits[i];
+            run_m += win_misses[i];
+        }
+        const uint32_t n = run_h + run_m;
+#pragma unroll
+        for (uint32_t k = 0; k < WIN; ++k) {
+            const uint64_t bh = __ballot(hk[k]), bm = __ballot(mk[k]);
+            if (hk[k]) win[off_h[k] + __builtin_amdgcn_mbcnt_hi((uint32_t)(bh >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bh, 0u))] = pk[k];
+            if (mk[k])
+                win[run_h + off_m[k] + __builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u))] = pk[k];
+        }
+        __syncthreads();
+        for (uint32_t k = 0; k < WIN; ++k) {
+            const uint32_t idx = k * WAVE_BLOCK + threadIdx.x;
+            bool cont = false, shadow = false;
+            uint32_t p = 0;
+            if (idx < n) {
+                p = win[idx];
+                const uint32_t ji = MULTI ? ps.job[p] : 0u;
+                const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, ji);
+                const uint32_t max_bounces = job_of<MULTI>(j0, jobs, ji).max_bounces;
+                const bool first = bounce == 0u;  // camera ray: state implied (see k_generate)
+                const f3 ray_origin = first ? camera_origin(s) : ld3(ps.ox, ps.oy, ps.oz, p);
+                const f3 ray_direction = ld3(ps.dx, ps.dy, ps.dz, p);
+                f3 throughput = first ? f3{1.0f, 1.0f, 1.0f} : ld3(ps.tr, ps.tg, ps.tb, p);
+                f3 L = first ? f3{0.0f, 0.0f, 0.0f} : ld3(ps.lr, ps.lg, ps.lb, p);
+                const float previous_bsdf_pdf = first ? 0.0f : ps.prev_pdf[p];
+                Rng rng{ps.rng[p]};
+                const int32_t hit = ps.hit[p];
+                if (hit == -1) {
+                    float mis = 1.0f;
+                    if (bounce > 0u)
+                        mis = power_heuristic(previous_bsdf_pdf, skybox_light_pdf(s, ray_direction, s.uniforms.light_probabilities[1]));
+                    L = L + throughput * evaluate_skybox(s, ray_direction) * mis;
+                } else {
+                    SurfaceHit sf;
+                    Bsdf b;
+                    f3 emission;
+                    float emission_mis;
+                    setup_surface(s, (uint32_t)hit, ps.ht[p], ps.hv[p], ps.hw[p], ray_origin, ray_direction, bounce,
+                                  previous_bsdf_pdf, rng, sf, b, emission, emission_mis);
+                    L = L + throughput * emission * emission_mis;
+                    const f3 next_origin = sf.hit_point + sf.flat_N * fmaxf(1e-4f, 1e-4f * ptm::length(sf.hit_point));
+                    LightSample light = sample_light(s, next_origin, rng);
+                    const f3 lwi = {ptm::dot(sf.Tt, light.direction), ptm::dot(sf.Bt, light.direction), ptm::dot(sf.N, light.direction)};
+                    if (light.pdf > 0.0f && lwi.z > 0.0f && ptm::any_pos(light.emission)) {
                         f3 f;
                         float pdf;
-                        evaluate_lobes(b, sf.wo, wi, f, pdf);
-                        if (pdf > 0.0f) {
-                            throughput = throughput * (f * wi.z / pdf);
-                            cont = true;
-                            if (bounce > 3u) {
-                                const float p_survive =
-                                    ptm::clamp_(fmaxf(throughput.x, fmaxf(throughput.y, throughput.z)), 0.05f, 0.95f);
-                                if (rng.next() > p_survive) cont = false;
-                                else throughput = throughput / p_survive;
-                            }
-                            if (cont) {
-                                st3(ps.ox, ps.oy, ps.oz, p, next_origin);
-                                st3(ps.dx, ps.dy, ps.dz, p, wi.x * sf.Tt + wi.y * sf.Bt + wi.z * sf.N);
-                                st3(ps.tr, ps.tg, ps.tb, p, throughput);
-                                ps.prev_pdf[p] = pdf;
-                                ps.rng[p] = rng.s;
+                        evaluate_lobes(b, sf.wo, lwi, f, pdf);
+                        const float weight = bounce < max_bounces ? power_heuristic(light.pdf, pdf) : 1.0f;
+                        const f3 contrib = throughput * f * lwi.z * light.emission * (weight / light.pdf);
+                        if (INLINE_SHADOW && !s.all_opaque) {
+                            TraversalCounters ctr = {0, 0};
+                            if (!occluded<false>(s, next_origin, light.direction, light.distance, light.triangle_index, rng, st, ctr))
+                                L = L + contrib;
+                        } else {
+                            shadow = true;
+                            st3(ps.sox, ps.soy, ps.soz, p, next_origin);
+                            st3(ps.sdx, ps.sdy, ps.sdz, p, light.direction);
+                            ps.sdist[p] = light.distance;
+                            ps.slight[p] = light.triangle_index;
+                            st3(ps.scr, ps.scg, ps.scb, p, contrib);
+                        }
+                    }
+                    if (bounce < max_bounces) {
+                        const f3 wi = sample_bsdf(b, sf.wo, rng);
+                        if (wi.z > 0.0f) {
+                            f3 f;
+                            float pdf;
+                            evaluate_lobes(b, sf.wo, wi, f, pdf);
+                            if (pdf > 0.0f) {
+                                throughput = throughput * (f * wi.z / pdf);
+                                cont = true;
+                                if (bounce > 3u) {
+                                    const float p_survive =
+                                        ptm::clamp_(fmaxf(throughput.x, fmaxf(throughput.y, throughput.z)), 0.05f, 0.95f);
+                                    if (rng.next() > p_survive) cont = false;
+                                    else throughput = throughput / p_survive;
+                                }
+                                if (cont) {
+                                    st3(ps.ox, ps.oy, ps.oz, p, next_origin);
+                                    st3(ps.dx, ps.dy, ps.dz, p, wi.x * sf.Tt + wi.y * sf.Bt + wi.z * sf.N);
+                                    st3(ps.tr, ps.tg, ps.tb, p, throughput);
+                                    ps.prev_pdf[p] = pdf;
+                                    ps.rng[p] = rng.s;
+                                }
                             }
                         }
                     }
                 }
+                st3(ps.lr, ps.lg, ps.lb, p, L);
             }
-            st3(ps.lr, ps.lg, ps.lb, p, L);
+            bq_push(nq, cont, p);
+            bq_push(sq, shadow, p);
+            bq_flush(nq, next_count, next_queue, false);
+            bq_flush(sq, shadow_count, shadow_queue, false);
         }
-        bq_push(nq, cont, p);
-        bq_push(sq, shadow, p);
-        bq_flush(nq, next_count, next_queue, false);
-        bq_flush(sq, shadow_count, shadow_queue, false);
     }
     bq_flush(nq, next_count, next_queue, true);
     bq_flush(sq, shadow_count, shadow_queue, true);
