@@ -46,24 +46,36 @@ def percentile(xs, q):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
-def cpu_baseline(world, w, h, bounces, samples, repeats):
+def cpu_baseline(world, w, h, bounces, samples, repeats, seconds_per_run=4.0):
     """The CPU oracle (C++ restatement of the reference method, OpenMP over
-    pixels) on the host cores: the benchmark's whole job (full frame at
-    `samples` spp), `repeats` times; the median run is reported."""
+    pixels) on the host cores, on a bounded sample of the benchmark's job: the
+    whole frame when one run takes at most about `seconds_per_run` (the 1080p
+    job), else a central band of full-width rows sized to that time from a
+    short probe, at the job's spp and depth; `repeats` runs, the median is
+    reported."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import binding as oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    probe = min(h, max(1, threads))
+    t0 = time.perf_counter()
+    oracle.render_rows_f32(world.desc, w, h, bounces, samples, (h - probe) // 2, (h - probe) // 2 + probe, threads=threads)
+    per_row = (time.perf_counter() - t0) / probe
+    rows = min(h, max(probe, int(seconds_per_run / max(per_row, 1e-9))))
+    row0 = (h - rows) // 2
     runs = []
-    for _ in range(max(1, repeats)):
+    for k in range(max(1, repeats)):
         t0 = time.perf_counter()
-        _, st = oracle.render_rows_f32(world.desc, w, h, bounces, samples, 0, h, threads=threads)
+        _, st = oracle.render_rows_f32(world.desc, w, h, bounces, samples, row0, row0 + rows, threads=threads)
         runs.append((time.perf_counter() - t0, st))
+        print(f"[bench] CPU baseline run {k + 1}: {runs[-1][0]:.2f} s", file=sys.stderr, flush=True)
     runs.sort(key=lambda r: r[0])
     dt, st = runs[len(runs) // 2]
     total = sum(r[0] for r in runs)
-    return {"value": w * h * samples / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
-            "sample": f"the benchmark job ({w}x{h}, {samples} spp, depth {bounces}) run {len(runs)} times, "
+    what = (f"the benchmark job ({w}x{h}" if rows == h else
+            f"a central band of the benchmark job's frame (rows {row0}-{row0 + rows - 1} of {w}x{h}")
+    return {"value": w * rows * samples / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
+            "sample": f"{what}, {samples} spp, depth {bounces}) run {len(runs)} times, "
                       f"median {dt:.2f} s of {total:.1f} s total; rays/s "
                       f"{((st['closest_rays'] + st['shadow_rays']) / dt):.4g}"}
 
