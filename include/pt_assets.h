@@ -8,7 +8,7 @@
  * (world.rs:1872-1873). The arrays come out in the reference's input order,
  * so pt_world_build then builds the BVH, alias tables and camera as the
  * reference does. Supported inputs: .glb (or .gltf with embedded data URIs)
- * with PNG images; sky images in Radiance .hdr, scanline OpenEXR (NONE, ZIPS, ZIP, PIZ) or PNG.
+ * with PNG or JPEG images; sky images in Radiance .hdr, scanline OpenEXR (NONE, ZIPS, ZIP, PIZ), PNG or JPEG.
  * Also the output-image writers of the `render` command (PNG, JPEG, WebP).
  */
 #ifndef PT_ASSETS_H

@@ -523,9 +523,11 @@ bool load_scene(const uint8_t* data, size_t size, Assets& A, std::string& err) {
                 n = v.len;
             }
             DecodedImage img;
-            if (n >= 3 && p[0] == 0xFF && p[1] == 0xD8 && p[2] == 0xFF)
-                return fail(err, "image " + std::to_string(ii) + " is JPEG, which this build does not decode (use PNG)");
-            if (!decode_png(p, n, img, err)) return fail(err, "image " + std::to_string(ii) + ": " + err);
+            const bool jpeg = n >= 3 && p[0] == 0xFF && p[1] == 0xD8 && p[2] == 0xFF;
+            if (n >= 12 && std::memcmp(p, "RIFF", 4) == 0 && std::memcmp(p + 8, "WEBP", 4) == 0)
+                return fail(err, "image " + std::to_string(ii) + " is WebP, which this loader does not decode (use PNG or JPEG)");
+            if (!(jpeg ? decode_jpeg(p, n, img, err) : decode_png(p, n, img, err)))
+                return fail(err, "image " + std::to_string(ii) + ": " + err);
             if (img.width > ATLAS || img.height > ATLAS) return fail(err, "single image is larger than the 8192 atlas");
             std::vector<uint8_t> rgba;
             to_rgba8(img, rgba);

@@ -1,6 +1,6 @@
 // image_io.cpp - decoders for the image files the reference's front end reads:
 // PNG (glTF textures and sky images) and Radiance RGBE .hdr (sky images);
-// OpenEXR skies are in exr_decode.cpp.
+// OpenEXR skies are in exr_decode.cpp, JPEG images in jpeg_decode.cpp.
 //
 // The reference decodes through the `image` crate (Cargo.toml: image 0.25,
 // via gltf::import_slice for textures and image::load_from_memory for the
@@ -230,9 +230,11 @@ bool decode_hdr(const uint8_t* data, size_t size, FloatImage& out, std::string& 
 
 bool decode_sky_image(const uint8_t* data, size_t size, FloatImage& out, std::string& err) {
     if (size >= 2 && data[0] == '#' && data[1] == '?') return decode_hdr(data, size, out, err);
-    if (size >= 8 && data[0] == 137 && data[1] == 'P') {
+    const bool png = size >= 8 && data[0] == 137 && data[1] == 'P';
+    const bool jpeg = size >= 3 && data[0] == 0xFF && data[1] == 0xD8 && data[2] == 0xFF;
+    if (png || jpeg) {
         DecodedImage img;
-        if (!decode_png(data, size, img, err)) return false;
+        if (!(png ? decode_png(data, size, img, err) : decode_jpeg(data, size, img, err))) return false;
         out.width = img.width;
         out.height = img.height;
         out.rgba.assign((size_t)img.width * img.height * 4, 0.0f);
@@ -253,7 +255,7 @@ bool decode_sky_image(const uint8_t* data, size_t size, FloatImage& out, std::st
     }
     if (size >= 4 && data[0] == 0x76 && data[1] == 0x2f && data[2] == 0x31 && data[3] == 0x01)
         return decode_exr(data, size, out, err);
-    return fail(err, "unrecognised sky image format (supported: Radiance .hdr, OpenEXR, PNG)");
+    return fail(err, "unrecognised sky image format (supported: Radiance .hdr, OpenEXR, PNG, JPEG)");
 }
 
 }  // namespace ptcore

@@ -1,4 +1,4 @@
-// image_io.h - image decoders of the asset front end (PNG textures, sky images).
+// image_io.h - image decoders of the asset front end (PNG and JPEG textures, sky images).
 #pragma once
 
 #include <cstddef>
@@ -23,10 +23,12 @@ struct FloatImage {
 };
 
 bool decode_png(const uint8_t* data, size_t size, DecodedImage& out, std::string& err);
+// Baseline/extended/progressive Huffman JPEG, 1 or 3 components, to 8-bit grey or RGB.
+bool decode_jpeg(const uint8_t* data, size_t size, DecodedImage& out, std::string& err);
 bool decode_hdr(const uint8_t* data, size_t size, FloatImage& out, std::string& err);
 // Single-part scanline OpenEXR (NONE, ZIPS, ZIP, PIZ; HALF/FLOAT/UINT channels) to RGBA32F.
 bool decode_exr(const uint8_t* data, size_t size, FloatImage& out, std::string& err);
-// A sky image as image::load_from_memory(..).into_rgba32f() gives it (Radiance .hdr, OpenEXR or PNG).
+// A sky image as image::load_from_memory(..).into_rgba32f() gives it (Radiance .hdr, OpenEXR, PNG or JPEG).
 bool decode_sky_image(const uint8_t* data, size_t size, FloatImage& out, std::string& err);
 
 }  // namespace ptcore
