@@ -143,8 +143,10 @@ __global__ void k_zero_counts(uint32_t* counts) {
     if (threadIdx.x < MAX_COUNTS) counts[threadIdx.x] = 0u;
 }
 
-// unit of path i: the last unit whose offset is <= i
+// unit of path i: the last unit whose offset is <= i (a division when all
+// units have the same size, else a binary search over the offsets)
 __device__ __forceinline__ uint32_t find_unit(const ChunkUnits& u, uint32_t i) {
+    if (u.unit_paths) return i / u.unit_paths;
     uint32_t lo = 0, hi = u.n_units;  // offset[lo] <= i < offset[hi]
     while (hi - lo > 1u) {
         uint32_t mid = (lo + hi) >> 1;
@@ -1072,6 +1074,12 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
         L.units.offset = d_uoff + off_start[c];
         L.units.n_units = (uint32_t)(chunks[c].unit_end - chunks[c].unit_begin);
         L.units.n_paths = chunks[c].paths;
+        L.units.unit_paths = 64u * job_tab[ujob[chunks[c].unit_begin]].samples;
+        for (size_t u = chunks[c].unit_begin; u < chunks[c].unit_end; ++u)
+            if (64u * job_tab[ujob[u]].samples != L.units.unit_paths) {
+                L.units.unit_paths = 0;
+                break;
+            }
         L.any_alpha = any_alpha;
         if (multi) run_chunk<true>(w, L, max_bounces, counting, lg, ctr);
         else run_chunk<false>(w, L, max_bounces, counting, lg, ctr);

@@ -52,6 +52,7 @@ struct ChunkUnits {
     const uint32_t* offset;  // n_units + 1 entries, relative to the chunk
     uint32_t n_units;
     uint32_t n_paths;
+    uint32_t unit_paths;     // paths of every unit when they are all alike (one job, or equal spp), else 0
 };
 
 }  // namespace ptd
