@@ -53,8 +53,9 @@ pt_status Context::ensure_overflow(uint32_t depth, uint32_t grid_threads) {
     size_t need = (size_t)extra * grid_threads * sizeof(uint32_t);
     if (need <= ovf_bytes) return PT_OK;
     if (ovf) {
-        hipStreamSynchronize(stream);
-        hipFree(ovf);
+        hipError_t se = hipStreamSynchronize(stream);
+        if (se != hipSuccess) return hip_error(se, "traversal overflow stack");
+        (void)hipFree(ovf);
         ovf = nullptr;
         ovf_bytes = 0;
     }
@@ -209,17 +210,19 @@ pt_status pt_context_create(int device, pt_context** out) {
     return PT_OK;
 }
 
+// Destruction releases everything; a failure of the work still pending on
+// the stream is reported after the release (the releases' own results are not).
 pt_status pt_context_destroy(pt_context* c) {
     if (!c) return PT_OK;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
-    hipFree(c->ovf);
-    hipFree(c->srgb_lut);
-    hipFree(c->counters);
+    (void)hipSetDevice(c->device);
+    const hipError_t e = c->stream ? hipStreamSynchronize(c->stream) : hipSuccess;
+    (void)hipFree(c->ovf);
+    (void)hipFree(c->srgb_lut);
+    (void)hipFree(c->counters);
     delete c->batch_wave;
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
-    return PT_OK;
+    return e == hipSuccess ? PT_OK : hip_error(e, "work pending at context destruction");
 }
 
 pt_status pt_context_synchronize(pt_context* c) {
@@ -234,7 +237,7 @@ pt_status pt_scene_create(pt_context* c, const pt_scene_desc* d, pt_scene** out)
     uint32_t depth = 0;
     pt_status st = validate_scene(d, depth);
     if (st != PT_OK) return st;
-    hipSetDevice(c->device);
+    if (pt_status dst = use_device(c->device)) return dst;
     pt_scene* s = new pt_scene();
     s->ctx = c;
     s->bvh_depth = depth;
@@ -329,22 +332,22 @@ pt_status pt_scene_create(pt_context* c, const pt_scene_desc* d, pt_scene** out)
 
 pt_status pt_scene_destroy(pt_scene* s) {
     if (!s) return PT_OK;
-    hipSetDevice(s->ctx->device);
-    hipStreamSynchronize(s->ctx->stream);
+    (void)hipSetDevice(s->ctx->device);
+    const hipError_t pending = hipStreamSynchronize(s->ctx->stream);
     const ptd::DevScene& v = s->view;
-    hipFree((void*)v.geo);
-    hipFree((void*)v.attr);
-    hipFree((void*)v.mat);
-    hipFree((void*)v.bvh);
-    hipFree((void*)v.tnode);
-    hipFree((void*)v.srgb);
-    hipFree((void*)v.linear);
-    hipFree((void*)v.tri_opaque);
-    hipFree((void*)v.sky);
-    hipFree((void*)v.sky_alias);
-    hipFree((void*)v.scene_alias);
+    (void)hipFree((void*)v.geo);
+    (void)hipFree((void*)v.attr);
+    (void)hipFree((void*)v.mat);
+    (void)hipFree((void*)v.bvh);
+    (void)hipFree((void*)v.tnode);
+    (void)hipFree((void*)v.srgb);
+    (void)hipFree((void*)v.linear);
+    (void)hipFree((void*)v.tri_opaque);
+    (void)hipFree((void*)v.sky);
+    (void)hipFree((void*)v.sky_alias);
+    (void)hipFree((void*)v.scene_alias);
     delete s;
-    return PT_OK;
+    return pending == hipSuccess ? PT_OK : hip_error(pending, "work pending at scene destruction");
 }
 
 pt_status pt_scene_footprint_bytes(const pt_scene* s, uint64_t* bytes) {
@@ -367,13 +370,14 @@ pt_status pt_scene_update_materials(pt_scene* s, const uint32_t* idx, const PtMa
         if (m[i].alpha_mode != PT_ALPHA_MODE_OPAQUE && s->view.all_opaque)
             return set_error(PT_ERR_INVALID, "switching a material to alpha-tested needs a new scene");
     }
-    hipSetDevice(s->ctx->device);
+    if (pt_status dst = use_device(s->ctx->device)) return dst;
     for (uint32_t i = 0; i < n; ++i) {
         hipError_t e = hipMemcpyAsync((void*)(s->view.mat + idx[i]), &m[i], sizeof(PtMaterial), hipMemcpyHostToDevice,
                                       s->ctx->stream);
         if (e != hipSuccess) return hip_error(e, "material update");
     }
-    hipStreamSynchronize(s->ctx->stream);
+    hipError_t se = hipStreamSynchronize(s->ctx->stream);
+    if (se != hipSuccess) return hip_error(se, "material update");
     ++s->version;
     return PT_OK;
 }
@@ -417,7 +421,7 @@ pt_status pt_scene_update_lights(pt_scene* s, const PtSceneUniforms* u, const Pt
     if (!s || !u || (n && !alias)) return set_error(PT_ERR_INVALID, "null argument");
     pt_status st = validate_lights(s, u, alias, n);
     if (st) return st;
-    hipSetDevice(s->ctx->device);
+    if (pt_status dst = use_device(s->ctx->device)) return dst;
     if ((st = write_alias(s, alias, n))) return st;
     s->view.uniforms = *u;
     s->alias_count = n;
@@ -444,7 +448,7 @@ pt_status pt_scene_apply_updates(pt_scene* s, const pt_world_updates* u) {
         pt_status st = validate_lights(s, &uni, u->scene_alias_changed ? u->scene_alias : nullptr, n);
         if (st) return st;
     }
-    hipSetDevice(s->ctx->device);
+    if (pt_status dst = use_device(s->ctx->device)) return dst;
     if (u->material_count) {
         pt_status st = pt_scene_update_materials(s, u->material_indices, u->materials, u->material_count);
         if (st) return st;
@@ -466,7 +470,7 @@ pt_status pt_scene_intersect(pt_scene* s, const float* rays, uint32_t count, int
     if (!s || !rays || !hit || !t || !v || !w) return set_error(PT_ERR_INVALID, "null argument");
     if (count == 0) return PT_OK;
     Context* c = s->ctx;
-    hipSetDevice(c->device);
+    if (pt_status dst = use_device(c->device)) return dst;
     uint32_t grid = std::min<uint32_t>((count + 255) / 256, c->cu_count * 8);
     pt_status st = c->ensure_overflow(s->bvh_depth + 1, grid * 256);
     if (st) return st;
@@ -498,7 +502,7 @@ pt_status pt_scene_occluded(pt_scene* s, const float* rays, const float* tmax, c
     if (!s || !rays || !tmax || !skip || !occ) return set_error(PT_ERR_INVALID, "null argument");
     if (count == 0) return PT_OK;
     Context* c = s->ctx;
-    hipSetDevice(c->device);
+    if (pt_status dst = use_device(c->device)) return dst;
     uint32_t grid = std::min<uint32_t>((count + 255) / 256, c->cu_count * 8);
     pt_status st = c->ensure_overflow(s->bvh_depth + 1, grid * 256);
     if (st) return st;

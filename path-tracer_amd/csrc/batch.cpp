@@ -60,7 +60,7 @@ pt_status run_batch(pt_tracer* const* tracers, uint32_t count, const uint32_t* s
     pt_status st = batch_jobs(tracers, count, samples, jobs);
     if (st) return st;
     Context* c = tracers[0]->ctx;
-    hipSetDevice(c->device);
+    if (pt_status dst = use_device(c->device)) return dst;
     if (!c->batch_wave) {
         c->batch_wave = new Wavefront();
         c->batch_wave->ctx = c;
@@ -104,11 +104,12 @@ pt_status pt_trace_batch(pt_tracer* const* tracers, uint32_t count, const uint32
 pt_status pt_autotune_batch(pt_tracer* const* tracers, uint32_t count, const uint32_t* samples, pt_tune_result* out) {
     if (!tracers || !samples || !out || count == 0) return set_error(PT_ERR_INVALID, "null argument");
     Context* c = tracers[0]->ctx;
-    hipSetDevice(c->device);
+    if (pt_status dst = use_device(c->device)) return dst;
     auto trial = [&](const ptd::WaveConfig& cfg, float& ms) -> pt_status {
         c->wave_cfg = cfg;
         for (uint32_t i = 0; i < count; ++i) tracers[i]->params.sample_count = 0;
-        hipStreamSynchronize(c->stream);
+        hipError_t se = hipStreamSynchronize(c->stream);
+        if (se != hipSuccess) return hip_error(se, "autotune trial");
         auto t0 = std::chrono::steady_clock::now();
         pt_status st = run_batch(tracers, count, samples);
         if (st) return st;
@@ -245,7 +246,7 @@ struct pt_server {
     }
 
     void run() {
-        hipSetDevice(ctx->device);
+        (void)hipSetDevice(ctx->device);  // a failure surfaces as the first round's error
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
             admit();
@@ -333,7 +334,7 @@ pt_status pt_server_create(pt_context* c, const pt_server_config* cfg, pt_server
     *out = nullptr;
     if (cfg->max_active_jobs == 0) return set_error(PT_ERR_INVALID, "max_active_jobs must be positive");
     if (cfg->hbm_budget_bytes == 0) return set_error(PT_ERR_INVALID, "hbm_budget_bytes must be positive");
-    hipSetDevice(c->device);
+    if (pt_status dst = use_device(c->device)) return dst;
     pt_server* s = new pt_server();
     s->ctx = c;
     s->cfg = *cfg;
@@ -360,13 +361,13 @@ pt_status pt_server_destroy(pt_server* s) {
     s->cv_work.notify_all();
     s->cv_done.notify_all();
     if (s->worker.joinable()) s->worker.join();
-    hipSetDevice(s->ctx->device);
-    hipStreamSynchronize(s->stream);
+    (void)hipSetDevice(s->ctx->device);
+    const hipError_t pending = hipStreamSynchronize(s->stream);
     for (auto& kv : s->jobs)
-        if (kv.second.accum) hipFree(kv.second.accum);
-    hipStreamDestroy(s->stream);
+        if (kv.second.accum) (void)hipFree(kv.second.accum);
+    (void)hipStreamDestroy(s->stream);
     delete s;
-    return PT_OK;
+    return pending == hipSuccess ? PT_OK : hip_error(pending, "work pending at server destruction");
 }
 
 pt_status pt_server_submit(pt_server* s, const pt_render_request* r, uint64_t* id) {
@@ -441,9 +442,9 @@ pt_status pt_server_result(pt_server* s, uint64_t id, float* rgba) {
         s->bytes_in_use -= j.footprint;
         s->jobs.erase(it);
     }
-    hipSetDevice(s->ctx->device);
-    hipError_t e = hipMemcpy(rgba, buf, bytes, hipMemcpyDeviceToHost);
-    hipFree(buf);
+    hipError_t e = hipSetDevice(s->ctx->device);
+    if (e == hipSuccess) e = hipMemcpy(rgba, buf, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
     s->cv_work.notify_all();  // memory freed: more jobs may fit
     return e == hipSuccess ? PT_OK : hip_error(e, "job result copy");
 }

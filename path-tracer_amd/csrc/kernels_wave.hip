@@ -687,12 +687,12 @@ struct EventScope {
     EventScope(EventLog* l, int k, hipStream_t s) : log(l), kind(k), st(s) {
         if (log) {
             log->take(a, b);
-            hipEventRecord(a, st);
+            if (a && hipEventRecord(a, st) != hipSuccess) a = nullptr;
         }
     }
     ~EventScope() {
         if (log) {
-            hipEventRecord(b, st);
+            if (b && hipEventRecord(b, st) != hipSuccess) b = nullptr;
             log->pairs.push_back({kind, a, b});
         }
     }
@@ -801,10 +801,13 @@ void run_chunk(Wavefront& w, const Launch& L, uint32_t max_bounces, bool countin
 
 }  // namespace
 
+// Timing is best effort: an event that cannot be created or recorded leaves
+// its pair out of the sums instead of failing the render.
 void EventLog::take(hipEvent_t& a, hipEvent_t& b) {
     if (pool.size() < 2) {
-        hipEventCreate(&a);
-        hipEventCreate(&b);
+        a = b = nullptr;
+        if (hipEventCreate(&a) != hipSuccess) a = nullptr;
+        if (hipEventCreate(&b) != hipSuccess) b = nullptr;
         return;
     }
     a = pool.back();
@@ -814,45 +817,47 @@ void EventLog::take(hipEvent_t& a, hipEvent_t& b) {
 }
 void EventLog::reset() {
     for (auto& p : pairs) {
-        pool.push_back(p.a);
-        pool.push_back(p.b);
+        if (p.a) pool.push_back(p.a);
+        if (p.b) pool.push_back(p.b);
     }
     pairs.clear();
 }
 void EventLog::sum(float ms[EV_KINDS], uint32_t launches[EV_KINDS]) {
     for (int k = 0; k < EV_KINDS; ++k) ms[k] = 0.0f, launches[k] = 0;
-    if (!pairs.empty()) hipEventSynchronize(pairs.back().b);
     for (auto& p : pairs) {
         float t = 0.0f;
-        hipEventElapsedTime(&t, p.a, p.b);
+        if (!p.a || !p.b || hipEventSynchronize(p.b) != hipSuccess || hipEventElapsedTime(&t, p.a, p.b) != hipSuccess)
+            continue;
         ms[p.kind] += t;
         launches[p.kind] += 1;
     }
 }
 EventLog::~EventLog() {
     reset();
-    for (hipEvent_t e : pool) hipEventDestroy(e);
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
 }
 
+// Teardown has no caller to report to: failures of the releases are ignored.
 Wavefront::~Wavefront() {
     if (partner) {
-        hipStreamSynchronize(partner->own_stream);
+        (void)hipStreamSynchronize(partner->own_stream);
         delete partner;
     }
-    if (arena) hipFree(arena);
-    if (ovf) hipFree(ovf);
-    if (tables) hipFree(tables);
-    if (staging) hipHostFree(staging);
-    if (ev_start) hipEventDestroy(ev_start);
-    if (ev_done) hipEventDestroy(ev_done);
-    if (own_stream) hipStreamDestroy(own_stream);
+    if (arena) (void)hipFree(arena);
+    if (ovf) (void)hipFree(ovf);
+    if (tables) (void)hipFree(tables);
+    if (staging) (void)hipHostFree(staging);
+    if (ev_start) (void)hipEventDestroy(ev_start);
+    if (ev_done) (void)hipEventDestroy(ev_done);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
 }
 
 pt_status Wavefront::ensure_overflow(size_t need, hipStream_t stream) {
     if (need <= ovf_bytes) return PT_OK;
     if (ovf) {
-        hipStreamSynchronize(stream);
-        hipFree(ovf);
+        hipError_t se = hipStreamSynchronize(stream);
+        if (se != hipSuccess) return hip_error(se, "wavefront overflow stacks");
+        (void)hipFree(ovf);
         ovf = nullptr;
     }
     hipError_t e = hipMalloc(&ovf, need);
@@ -867,8 +872,9 @@ pt_status Wavefront::ensure_overflow(size_t need, hipStream_t stream) {
 pt_status Wavefront::reserve(uint32_t paths) {
     if (paths <= capacity && arena) return PT_OK;
     if (arena) {
-        hipStreamSynchronize(stream_override ? stream_override : ctx->stream);
-        hipFree(arena);
+        hipError_t se = hipStreamSynchronize(stream_override ? stream_override : ctx->stream);
+        if (se != hipSuccess) return hip_error(se, "wavefront path state");
+        (void)hipFree(arena);
         arena = nullptr;
         capacity = 0;
     }
@@ -1017,18 +1023,18 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
     const size_t b_scenes = scene_tab.size() * sizeof(DevScene), b_jobs = job_tab.size() * sizeof(WaveJob);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t bytes = al(b_scenes) + al(b_jobs) + al(ujob.size() * 4) + al(ublock.size() * 4) + al(off_packed.size() * 4);
+    // the staging buffer may still feed the previous render's table copy
+    hipError_t se = hipStreamSynchronize(stream);
+    if (se != hipSuccess) return hip_error(se, "wavefront job tables");
     if (bytes > tables_bytes) {
-        hipStreamSynchronize(stream);
-        if (tables) hipFree(tables);
-        if (staging) hipHostFree(staging);
+        if (tables) (void)hipFree(tables);
+        if (staging) (void)hipHostFree(staging);
         tables = staging = nullptr;
         tables_bytes = staging_bytes = 0;
         hipError_t e = hipMalloc(&tables, bytes);
         if (e == hipSuccess) e = hipHostMalloc(&staging, bytes, 0);
         if (e != hipSuccess) return hip_error(e, "wavefront job tables");
         tables_bytes = staging_bytes = bytes;
-    } else {
-        hipStreamSynchronize(stream);  // staging may still feed an earlier copy
     }
     char* hs = (char*)staging;
     char* ds = (char*)tables;
@@ -1050,12 +1056,14 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
     const bool counting = profiling >= 2;
     EventLog* lg = profiling >= 1 ? log : nullptr;
     unsigned long long* ctr = counting ? ctx->counters : nullptr;
-    if (counting) hipMemsetAsync(ctx->counters, 0, 16 * sizeof(unsigned long long), stream);
+    if (counting && (e = hipMemsetAsync(ctx->counters, 0, 16 * sizeof(unsigned long long), stream)) != hipSuccess)
+        return hip_error(e, "traversal counters");
     EventScope total(lg, EV_TOTAL, stream);
     const bool two = lanes == 2 && chunks.size() > 1;
     if (two) {  // the second lane starts after the tables are on the device
-        hipEventRecord(ev_start, stream);
-        hipStreamWaitEvent(partner->own_stream, ev_start, 0);
+        if ((e = hipEventRecord(ev_start, stream)) != hipSuccess ||
+            (e = hipStreamWaitEvent(partner->own_stream, ev_start, 0)) != hipSuccess)
+            return hip_error(e, "second lane start");
     }
     for (size_t c = 0; c < chunks.size(); ++c) {
         Wavefront& w = *lane[two ? c % 2 : 0];
@@ -1087,8 +1095,9 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
         if (e != hipSuccess) return hip_error(e, "wavefront launch");
         if (counting) {
             uint32_t h[MAX_COUNTS];
-            hipMemcpyAsync(h, w.counts, sizeof(h), hipMemcpyDeviceToHost, lane_stream);
-            hipStreamSynchronize(lane_stream);
+            if ((e = hipMemcpyAsync(h, w.counts, sizeof(h), hipMemcpyDeviceToHost, lane_stream)) != hipSuccess ||
+                (e = hipStreamSynchronize(lane_stream)) != hipSuccess)
+                return hip_error(e, "ray counts");
             for (uint32_t b = 0; b <= max_bounces; ++b) {
                 host_counters[0] += h[2 * b];
                 host_counters[1] += h[2 * b + 1];
@@ -1096,8 +1105,9 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
         }
     }
     if (two) {  // the caller's stream resumes after both lanes
-        hipEventRecord(ev_done, partner->own_stream);
-        hipStreamWaitEvent(stream, ev_done, 0);
+        if ((e = hipEventRecord(ev_done, partner->own_stream)) != hipSuccess ||
+            (e = hipStreamWaitEvent(stream, ev_done, 0)) != hipSuccess)
+            return hip_error(e, "second lane join");
     }
     return PT_OK;
 }

@@ -11,6 +11,11 @@
 namespace ptcore {
 
 pt_status hip_error(hipError_t e, const char* what);
+// Makes `device` current for the calling thread.
+inline pt_status use_device(int device) {
+    const hipError_t e = hipSetDevice(device);
+    return e == hipSuccess ? PT_OK : hip_error(e, "hipSetDevice");
+}
 
 // One HIP device and stream (the reference's wgpu device + queue).
 struct Context {
@@ -51,7 +56,7 @@ struct DeviceBuffer {
         return PT_OK;
     }
     ~DeviceBuffer() {
-        if (ptr) hipFree(ptr);
+        if (ptr) (void)hipFree(ptr);  // teardown: nothing to report
     }
 };
 

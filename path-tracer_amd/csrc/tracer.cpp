@@ -48,8 +48,8 @@ HostJob tracer_job(pt_tracer* t, uint32_t samples) {
 namespace {
 
 pt_status alloc_tile(pt_tracer* t, uint32_t w, uint32_t h) {
-    hipFree(t->accum);
-    hipFree(t->storage);
+    (void)hipFree(t->accum);
+    (void)hipFree(t->storage);
     t->accum = t->storage = nullptr;
     size_t n = (size_t)w * h * sizeof(float4);
     hipError_t e = hipMalloc(&t->accum, n);
@@ -89,7 +89,7 @@ ptd::TileArgs tile_args(const pt_tracer* t, uint32_t first_sample, uint32_t samp
 
 pt_status trace_samples(pt_tracer* t, uint32_t samples) {
     Context* c = t->ctx;
-    hipSetDevice(c->device);
+    if (pt_status dst = use_device(c->device)) return dst;
     sync_version(t);
     const uint32_t first = t->params.sample_count + 1u;
     ptd::TileArgs a = tile_args(t, first, samples);
@@ -99,23 +99,27 @@ pt_status trace_samples(pt_tracer* t, uint32_t samples) {
         const uint32_t grid = std::min<uint32_t>(blocks, c->cu_count * 8u);
         if ((st = c->ensure_overflow(t->scene->bvh_depth + 1u, grid * 256u))) return st;
         const bool counting = t->profiling >= 2;
-        if (counting) hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream);
+        if (counting) {
+            hipError_t me = hipMemsetAsync(c->counters, 0, 16 * sizeof(unsigned long long), c->stream);
+            if (me != hipSuccess) return hip_error(me, "traversal counters");
+        }
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (t->profiling) {
             t->log.take(e0, e1);
-            hipEventRecord(e0, c->stream);
+            if (e0 && hipEventRecord(e0, c->stream) != hipSuccess) e0 = nullptr;  // timing is best effort
         }
         hipError_t e = launch_megakernel(t->scene->view, a, t->accum, t->storage, c->ovf, grid,
                                          counting ? c->counters : nullptr, c->stream);
         if (e != hipSuccess) return hip_error(e, "megakernel launch");
         if (t->profiling) {
-            hipEventRecord(e1, c->stream);
+            if (e1 && hipEventRecord(e1, c->stream) != hipSuccess) e1 = nullptr;
             t->log.pairs.push_back({EV_TOTAL, e0, e1});
         }
         if (counting) {
             unsigned long long h[4];
-            hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream);
-            hipStreamSynchronize(c->stream);
+            if ((e = hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream)) != hipSuccess ||
+                (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+                return hip_error(e, "traversal counters");
             for (int k = 0; k < 4; ++k) t->counters[k] += h[k];
         }
     } else {
@@ -126,8 +130,9 @@ pt_status trace_samples(pt_tracer* t, uint32_t samples) {
         if ((st = t->wave->render({tracer_job(t, samples)}, t->profiling, &t->log, t->counters))) return st;
         if (t->profiling >= 2) {
             unsigned long long h[6];
-            hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream);
-            hipStreamSynchronize(c->stream);
+            hipError_t e = hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) return hip_error(e, "traversal counters");
             for (int k = 2; k < 6; ++k) t->counters[k] += h[k];
         }
     }
@@ -145,7 +150,7 @@ pt_status pt_tracer_create(pt_context* c, pt_scene* s, uint32_t tile_w, uint32_t
     if (s->ctx != c) return set_error(PT_ERR_INVALID, "scene belongs to another context");
     pt_status st = validate_tile(tile_w, tile_h);
     if (st) return st;
-    hipSetDevice(c->device);
+    if (pt_status dst = use_device(c->device)) return dst;
     pt_tracer* t = new pt_tracer();
     t->ctx = c;
     t->scene = s;
@@ -163,13 +168,13 @@ pt_status pt_tracer_create(pt_context* c, pt_scene* s, uint32_t tile_w, uint32_t
 
 pt_status pt_tracer_destroy(pt_tracer* t) {
     if (!t) return PT_OK;
-    hipSetDevice(t->ctx->device);
-    hipStreamSynchronize(t->ctx->stream);
+    (void)hipSetDevice(t->ctx->device);
+    const hipError_t pending = hipStreamSynchronize(t->ctx->stream);
     delete t->wave;
-    hipFree(t->accum);
-    hipFree(t->storage);
+    (void)hipFree(t->accum);
+    (void)hipFree(t->storage);
     delete t;
-    return PT_OK;
+    return pending == hipSuccess ? PT_OK : hip_error(pending, "work pending at tracer destruction");
 }
 
 pt_status pt_tracer_set_schedule(pt_tracer* t, int schedule) {
@@ -186,9 +191,10 @@ pt_status pt_tracer_set_tile(pt_tracer* t, uint32_t iw, uint32_t ih, uint32_t ox
     if (ox >= iw || oy >= ih) return set_error(PT_ERR_INVALID, "tile origin must be within the image");
     pt_status st = validate_tile(tw, th);
     if (st) return st;
-    hipSetDevice(t->ctx->device);
+    if (pt_status dst = use_device(t->ctx->device)) return dst;
     if (tw != t->tile_w || th != t->tile_h) {
-        hipStreamSynchronize(t->ctx->stream);
+        hipError_t se = hipStreamSynchronize(t->ctx->stream);
+        if (se != hipSuccess) return hip_error(se, "tile resize");
         if ((st = alloc_tile(t, tw, th))) return st;
     }
     t->params.image_size[0] = iw;
@@ -238,7 +244,7 @@ pt_status pt_tracer_trace_all(pt_tracer* t, uint32_t samples) {
 
 pt_status pt_tracer_read_accumulation(pt_tracer* t, float* rgba) {
     if (!t || !rgba) return set_error(PT_ERR_INVALID, "null argument");
-    hipSetDevice(t->ctx->device);
+    if (pt_status dst = use_device(t->ctx->device)) return dst;
     hipError_t e = hipMemcpyAsync(rgba, t->accum, (size_t)t->tile_w * t->tile_h * sizeof(float4), hipMemcpyDeviceToHost,
                                   t->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(t->ctx->stream);
@@ -247,7 +253,7 @@ pt_status pt_tracer_read_accumulation(pt_tracer* t, float* rgba) {
 
 pt_status pt_tracer_write_accumulation(pt_tracer* t, const float* rgba) {
     if (!t || !rgba) return set_error(PT_ERR_INVALID, "null argument");
-    hipSetDevice(t->ctx->device);
+    if (pt_status dst = use_device(t->ctx->device)) return dst;
     hipError_t e = hipMemcpyAsync(t->accum, rgba, (size_t)t->tile_w * t->tile_h * sizeof(float4), hipMemcpyHostToDevice,
                                   t->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(t->ctx->stream);
@@ -256,7 +262,7 @@ pt_status pt_tracer_write_accumulation(pt_tracer* t, const float* rgba) {
 
 pt_status pt_tracer_read_storage(pt_tracer* t, float* rgba) {
     if (!t || !rgba) return set_error(PT_ERR_INVALID, "null argument");
-    hipSetDevice(t->ctx->device);
+    if (pt_status dst = use_device(t->ctx->device)) return dst;
     hipError_t e = hipMemcpyAsync(rgba, t->storage, (size_t)t->tile_w * t->tile_h * sizeof(float4), hipMemcpyDeviceToHost,
                                   t->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(t->ctx->stream);
@@ -265,7 +271,7 @@ pt_status pt_tracer_read_storage(pt_tracer* t, float* rgba) {
 
 pt_status pt_tracer_display_rgba8(pt_tracer* t, uint8_t* rgba) {
     if (!t || !rgba) return set_error(PT_ERR_INVALID, "null argument");
-    hipSetDevice(t->ctx->device);
+    if (pt_status dst = use_device(t->ctx->device)) return dst;
     const uint32_t n = t->tile_w * t->tile_h;
     DeviceBuffer buf;
     pt_status st = buf.alloc((size_t)n * 4);
@@ -280,7 +286,7 @@ pt_status pt_tracer_primary_hits(pt_tracer* t, uint32_t sample_count, int32_t* t
     if (!t || !tri || !tt) return set_error(PT_ERR_INVALID, "null argument");
     if (sample_count == 0) return set_error(PT_ERR_INVALID, "sample_count is 1-based");
     Context* c = t->ctx;
-    hipSetDevice(c->device);
+    if (pt_status dst = use_device(c->device)) return dst;
     const uint32_t n = t->tile_w * t->tile_h;
     const uint32_t blocks = ((t->tile_w + 15u) / 16u) * ((t->tile_h + 15u) / 16u);
     const uint32_t grid = std::min<uint32_t>(blocks, c->cu_count * 8u);
@@ -301,8 +307,9 @@ pt_status pt_tracer_primary_hits(pt_tracer* t, uint32_t sample_count, int32_t* t
 pt_status pt_tracer_set_profiling(pt_tracer* t, int level) {
     if (!t) return set_error(PT_ERR_INVALID, "null tracer");
     if (level < 0 || level > 2) return set_error(PT_ERR_INVALID, "profiling level %d not in 0..2", level);
-    hipSetDevice(t->ctx->device);
-    hipStreamSynchronize(t->ctx->stream);
+    if (pt_status dst = use_device(t->ctx->device)) return dst;
+    hipError_t se = hipStreamSynchronize(t->ctx->stream);
+    if (se != hipSuccess) return hip_error(se, "profiling switch");
     t->profiling = level;
     t->log.reset();
     std::memset(t->counters, 0, sizeof(t->counters));
@@ -311,7 +318,7 @@ pt_status pt_tracer_set_profiling(pt_tracer* t, int level) {
 
 pt_status pt_tracer_stats(pt_tracer* t, pt_trace_stats* out) {
     if (!t || !out) return set_error(PT_ERR_INVALID, "null argument");
-    hipSetDevice(t->ctx->device);
+    if (pt_status dst = use_device(t->ctx->device)) return dst;
     float ms[EV_KINDS];
     uint32_t launches[EV_KINDS];
     t->log.sum(ms, launches);
