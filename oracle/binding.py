@@ -118,12 +118,12 @@ def render_rows_f32(desc, image_w, image_h, max_bounces, samples, row_begin, row
     return acc, {k: getattr(st, k) for k, _ in OracleStats._fields_}
 
 
-def render_f64(desc, image_w, image_h, max_bounces, samples, first_sample=1, threads=0):
-    acc = np.zeros((image_h, image_w, 4), np.float64)
+def render_f64(desc, image_w, image_h, max_bounces, samples, first_sample=1, threads=0, tile=None, origin=(0, 0)):
+    tw, th = tile or (image_w, image_h)
+    acc = np.zeros((th, tw, 4), np.float64)
     st = OracleStats()
-    p = params(image_w, image_h, max_bounces, first_sample)
-    rc = lib().oracle_render_f64(C.byref(desc), C.byref(p), image_w, image_h, samples, _ptr(acc), threads,
-                                 C.byref(st))
+    p = params(image_w, image_h, max_bounces, first_sample, origin)
+    rc = lib().oracle_render_f64(C.byref(desc), C.byref(p), tw, th, samples, _ptr(acc), threads, C.byref(st))
     assert rc == 0
     return acc
 
