@@ -26,13 +26,38 @@ namespace ptcore {
 
 static thread_local std::string g_error;
 
+// Messages can quote bytes of the caller's files (a corrupt header line, a
+// JSON string). Whatever is not well-formed UTF-8 (including a sequence cut
+// by the length limit) becomes '?', so the message is always valid text.
+std::string utf8_clean(const char* s) {
+    std::string out;
+    const unsigned char* p = (const unsigned char*)s;
+    while (*p) {
+        const unsigned char c = *p;
+        int n = c < 0x80 ? 0 : (c >> 5) == 6 ? 1 : (c >> 4) == 14 ? 2 : (c >> 3) == 30 ? 3 : -1;
+        bool ok = n >= 0;
+        for (int k = 1; ok && k <= n; ++k) ok = (p[k] & 0xC0) == 0x80;
+        if (ok && n == 1) ok = c >= 0xC2;                                  // no overlong 2-byte forms
+        if (ok && n == 2) ok = !(c == 0xE0 && p[1] < 0xA0) && !(c == 0xED && p[1] >= 0xA0);  // overlong, surrogates
+        if (ok && n == 3) ok = c <= 0xF4 && !(c == 0xF0 && p[1] < 0x90) && !(c == 0xF4 && p[1] >= 0x90);
+        if (ok && (c >= 0x20 || c == '\n' || c == '\t')) {
+            out.append((const char*)p, (size_t)n + 1);
+            p += n + 1;
+        } else {
+            out.push_back('?');
+            ++p;
+        }
+    }
+    return out;
+}
+
 pt_status set_error(pt_status code, const char* fmt, ...) {
     char buf[1024];
     va_list ap;
     va_start(ap, fmt);
     vsnprintf(buf, sizeof(buf), fmt, ap);
     va_end(ap);
-    g_error = buf;
+    g_error = utf8_clean(buf);
     return code;
 }
 

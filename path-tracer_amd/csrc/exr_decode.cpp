@@ -410,6 +410,8 @@ bool decode_exr(const uint8_t* data, size_t size, FloatImage& out, std::string& 
     if (chans.empty() || !have_dw) return fail(err, "header without channels or data window");
     const int64_t w = (int64_t)dw[2] - dw[0] + 1, h = (int64_t)dw[3] - dw[1] + 1;
     if (w <= 0 || h <= 0 || w > 65536 || h > 65536) return fail(err, "bad data window");
+    if ((uint64_t)w * (uint64_t)h > MAX_IMAGE_PIXELS)
+        return fail(err, "image of " + std::to_string(w) + "x" + std::to_string(h) + " exceeds the image size limit");
     int lines_per_chunk;
     switch (compression) {
         case 0: case 2: lines_per_chunk = 1; break;   // NONE, ZIPS

@@ -777,6 +777,7 @@ pt_status pt_assets_load_scene(pt_assets* a, const uint8_t* bytes, size_t size) 
     } catch (const std::bad_alloc&) {
         return set_error(PT_ERR_MEMORY, "out of host memory while loading the scene");
     }
+    for (std::string& w : fresh.warnings) w = utf8_clean(w.c_str());  // they can quote the file's JSON
     fresh.sky = std::move(a->sky);
     static_cast<Assets&>(*a) = std::move(fresh);
     return PT_OK;

@@ -89,7 +89,8 @@ bool decode_png(const uint8_t* data, size_t size, DecodedImage& out, std::string
                           (ctype == 3 && (depth == 1 || depth == 2 || depth == 4 || depth == 8)) ||
                           ((ctype == 2 || ctype == 4 || ctype == 6) && (depth == 8 || depth == 16));
     if (!depth_ok) return fail(err, "bad PNG bit depth");
-    if ((uint64_t)w * h > (1ull << 28)) return fail(err, "PNG too large");
+    if ((uint64_t)w * h > MAX_IMAGE_PIXELS)
+        return fail(err, "PNG of " + std::to_string(w) + "x" + std::to_string(h) + " exceeds the image size limit");
     const size_t row_bits = (size_t)w * channels * depth;
     const size_t row_bytes = (row_bits + 7) / 8;
     const size_t bpp = std::max<size_t>(1, (size_t)channels * depth / 8);
@@ -182,6 +183,8 @@ bool decode_hdr(const uint8_t* data, size_t size, FloatImage& out, std::string& 
         std::strcmp(xa, "+X") != 0)
         return fail(err, "unsupported HDR orientation '" + s + "'");
     if (ww <= 0 || hh <= 0 || ww > 65535 || hh > 65535) return fail(err, "bad HDR size");
+    if ((uint64_t)ww * (uint64_t)hh > MAX_IMAGE_PIXELS)
+        return fail(err, "HDR image of " + std::to_string(ww) + "x" + std::to_string(hh) + " exceeds the image size limit");
     const uint32_t w = (uint32_t)ww, h = (uint32_t)hh;
     out.width = w;
     out.height = h;

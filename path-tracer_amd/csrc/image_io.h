@@ -8,6 +8,12 @@
 
 namespace ptcore {
 
+// Largest image any decoder accepts (8192 x 8192 pixels), checked against the
+// header before anything is allocated: a small file cannot make the loader
+// allocate or process an arbitrarily large image. (The reference's `image`
+// crate refuses decodes beyond its 512 MiB default allocation limit.)
+constexpr uint64_t MAX_IMAGE_PIXELS = 1ull << 26;
+
 // Decoded integer image: `channels` interleaved samples per pixel, 8 or 16
 // significant bits each (palettes expanded to RGB/RGBA, low-bit grey scaled to 8).
 struct DecodedImage {

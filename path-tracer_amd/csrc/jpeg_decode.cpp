@@ -224,6 +224,8 @@ struct Decoder {
         width = p[3] << 8 | p[4];
         const int nc = p[5];
         if (width == 0 || height == 0) return fail(err, "zero image size (DNL is not supported)");
+        if ((uint64_t)width * (uint64_t)height > MAX_IMAGE_PIXELS)
+            return fail(err, "image of " + std::to_string(width) + "x" + std::to_string(height) + " exceeds the image size limit");
         if (nc != 1 && nc != 3) return fail(err, std::to_string(nc) + "-component images are not supported");
         if (len < 6 + 3 * (size_t)nc) return fail(err, "truncated SOF");
         comp.resize(nc);

@@ -4,6 +4,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <map>
+#include <string>
 #include <vector>
 
 #include "../../include/pt_api.h"
@@ -12,6 +13,8 @@ namespace ptcore {
 
 // Thread-local error message for pt_last_error(); returns `code`.
 pt_status set_error(pt_status code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+// `s` with every byte that is not part of well-formed UTF-8 text replaced by '?'.
+std::string utf8_clean(const char* s);
 
 // Host-side prepared world (the arrays World::take_updates hands to the GPU).
 struct World {

@@ -284,7 +284,7 @@ def exported_symbols():
 
 def _check(code):
     if code != PT_OK:
-        raise PtError(code, lib().pt_last_error().decode())
+        raise PtError(code, lib().pt_last_error().decode("utf-8", "replace"))
 
 
 def _ptr(a):
@@ -542,7 +542,8 @@ class Assets:
         return {k: getattr(s, k) for k, _ in AssetsSummary._fields_}
 
     def warnings(self):
-        return [lib().pt_assets_warning(self.handle, i).decode() for i in range(self.info()["warning_count"])]
+        return [lib().pt_assets_warning(self.handle, i).decode("utf-8", "replace")
+                for i in range(self.info()["warning_count"])]
 
     def world_desc(self, skybox_luminance=-1.0, camera_position=None, yaw=None, pitch=None, fov=None, aspect=1.0):
         d = WorldDesc()

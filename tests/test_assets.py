@@ -368,3 +368,46 @@ def test_reference_4k_exr_sky_matches_its_2k_hdr():
     rel = np.abs(le - lh) / np.maximum(lh, 1e-6)
     assert np.median(rel) < 0.01 and np.percentile(rel, 99) < 0.03
     assert np.corrcoef(le.ravel(), lh.ravel())[0, 1] > 0.9999
+
+
+def test_huge_declared_sizes_are_rejected_before_allocation():
+    # a small file whose header declares a giant image must fail at once, not allocate or decode it
+    import struct
+    import time
+    import zlib
+
+    small = png(np.zeros((4, 4, 3), np.uint8))
+    big_png = bytearray(small)
+    ihdr = small.index(b"IHDR")
+    big_png[ihdr + 4:ihdr + 12] = struct.pack(">II", 20000, 20000)
+    big_png[ihdr + 17:ihdr + 21] = struct.pack(">I", zlib.crc32(bytes(big_png[ihdr:ihdr + 17])) & 0xFFFFFFFF)
+    big_hdr = b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 20000 +X 20000\n" + bytes(64)
+    small_exr = exr({"R": np.ones((2, 2), np.float32), "G": np.ones((2, 2), np.float32), "B": np.ones((2, 2), np.float32)})
+    dw = small_exr.index(b"dataWindow\0box2i\0") + len(b"dataWindow\0box2i\0") + 4
+    big_exr = small_exr[:dw] + struct.pack("<iiii", 0, 0, 19999, 19999) + small_exr[dw + 16:]
+    files = [bytes(big_png), big_hdr, big_exr]
+    try:
+        import io
+        from PIL import Image
+        buf = io.BytesIO()
+        Image.fromarray(np.zeros((16, 16, 3), np.uint8)).save(buf, "JPEG")
+        jpg = bytearray(buf.getvalue())
+        sof = jpg.index(b"\xff\xc0")
+        jpg[sof + 5:sof + 9] = struct.pack(">HH", 65535, 65407)  # height, width
+        files.append(bytes(jpg))
+    except ImportError:
+        pass
+    for data in files:
+        t0 = time.perf_counter()
+        with pytest.raises(ptcore.PtError, match="exceeds the image size limit"):
+            ptcore.Assets(quad_glb(), data)
+        assert time.perf_counter() - t0 < 1.0
+
+
+def test_error_messages_are_valid_text():
+    # the message quotes the file's bytes: invalid UTF-8 becomes '?', valid UTF-8 is kept
+    bad = b"#?RADIANCE\nFORMAT=\xf9\xfe\xc3(\xc3\xa9\n\n-Y 1 +X 1\n" + bytes(4)
+    with pytest.raises(ptcore.PtError) as e:
+        ptcore.Assets(quad_glb(), bad)
+    msg = str(e.value)
+    assert "unsupported HDR pixel format ???(é" in msg
