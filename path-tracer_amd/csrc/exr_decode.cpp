@@ -157,6 +157,8 @@ bool huf_uncompress(const uint8_t* in, size_t n_in, uint16_t* out, size_t n_out,
     // decode; symbol iM is the run code: the next 8 bits repeat the previous word
     const int rlc = (int)iM;
     size_t o = 0;
+    uint64_t scanned = 0;
+    const uint64_t scan_budget = 64ull * n_out + (1ull << 20);
     uint64_t c = 0;
     int lc = 0;
     const uint8_t* p = bits;
@@ -189,6 +191,10 @@ bool huf_uncompress(const uint8_t* in, size_t n_in, uint16_t* out, size_t n_out,
                 if (!emit(d.lit)) return fail(err, "Huffman data overrun");
             } else {
                 if (d.longs.empty()) return fail(err, "invalid Huffman code");
+                // codes longer than the table are found by a scan of their prefix's list; a table that
+                // makes those scans long (many long codes under one prefix) is refused instead of decoded
+                scanned += d.longs.size();
+                if (scanned > scan_budget) return fail(err, "Huffman table too costly to decode");
                 bool found = false;
                 for (int sym : d.longs) {
                     const int l = (int)(hcode[sym] & 63);
