@@ -17,25 +17,39 @@
 // Floating-point steps follow the reference's types (f32 with mul_add where
 // it uses mul_add, f64 where it widens), so the arrays come out bit-identical.
 #include <algorithm>
+#include <atomic>
 #include <cstddef>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <limits>
 #include <new>
+#include <stdexcept>
 #include <string>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "pt_internal.h"
 
 namespace {
 
+// fmin/fmax as the reference's f32::min/max: a NaN operand gives the other
+// one; of equal operands (+0 and -0) the second, as glibc's fminf/fmaxf give.
+// Written so the compiler emits minss/maxss behind a NaN test that is never
+// taken on real data: as library calls, or as compare-and-branch on the
+// values, they were most of the BVH build's time.
+inline float min_f(float x, float y) { return y != y ? x : (x < y ? x : y); }
+inline float max_f(float x, float y) { return y != y ? x : (x > y ? x : y); }
+
 struct Aabb {
     float mn[3] = {INFINITY, INFINITY, INFINITY};
     float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     void grow(const Aabb& o) {
         for (int k = 0; k < 3; ++k) {
-            mn[k] = std::fmin(mn[k], o.mn[k]);
-            mx[k] = std::fmax(mx[k], o.mx[k]);
+            mn[k] = min_f(mn[k], o.mn[k]);
+            mx[k] = max_f(mx[k], o.mx[k]);
         }
     }
 };
@@ -59,127 +73,466 @@ inline size_t f32_as_usize(float v) {
     return (size_t)v;
 }
 
-struct Builder {
-    std::vector<PtBvhNode>& nodes;
-    std::vector<BvhTri>& tris;
-
-    void set_box(size_t ni, size_t start, size_t end) {
-        Aabb b;
-        for (size_t i = start; i < end; ++i) b.grow(tris[i].box);
-        for (int k = 0; k < 3; ++k) {
-            nodes[ni].aabb_min[k] = b.mn[k];
-            nodes[ni].aabb_max[k] = b.mx[k];
-        }
+// Worker threads for host preparation: PT_HOST_THREADS when set (1 builds on
+// the calling thread alone), else the hardware's count capped at 16.
+unsigned host_threads() {
+    if (const char* v = std::getenv("PT_HOST_THREADS")) {
+        long t = std::strtol(v, nullptr, 10);
+        if (t >= 1) return (unsigned)std::min(t, 256L);
     }
+    unsigned hw = std::thread::hardware_concurrency();
+    return std::max(1u, std::min(hw, 16u));
+}
 
-    struct Item {
-        size_t node, start, end;
+// Triangles below which a BVH pass is not cut into chunks, and twice which a
+// subtree is not split further before the workers take over:
+// PT_BVH_TASK_GRAIN when set, else 4096.
+size_t bvh_task_grain() {
+    if (const char* v = std::getenv("PT_BVH_TASK_GRAIN")) {
+        long g = std::strtol(v, nullptr, 10);
+        if (g >= 1) return (size_t)g;
+    }
+    return 4096;
+}
+
+// Raised when a split sends every triangle to one side. The child would then
+// repeat its parent's range and box, and the reference's recursion never
+// ends (it overflows its stack): non-finite SAH costs from NaN or
+// overflowing vertex coordinates lead there.
+struct BvhNoProgress {};
+
+// Run f(t) for t in [0, threads) on as many threads, the caller taking t = 0;
+// the first exception any of them threw is rethrown after all have joined.
+template <class F>
+void run_parallel(unsigned threads, F&& f) {
+    if (threads <= 1) {
+        f(0u);
+        return;
+    }
+    std::vector<std::exception_ptr> errors(threads);
+    auto guarded = [&](unsigned t) {
+        try {
+            f(t);
+        } catch (...) {
+            errors[t] = std::current_exception();
+        }
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(threads - 1);
+    try {
+        for (unsigned t = 1; t < threads; ++t) pool.emplace_back(guarded, t);
+    } catch (...) {
+        for (std::thread& th : pool) th.join();
+        throw;
+    }
+    guarded(0u);
+    for (std::thread& th : pool) th.join();
+    for (std::exception_ptr& e : errors)
+        if (e) std::rethrow_exception(e);
+}
+
+// The BVH builder of world.rs:1430-1591: binned SAH over centroid bounds with
+// PT_BVH_BINS bins, leaf at <= 2 triangles or when the best split costs at
+// least the leaf, modified Hoare partition, children allocated as an adjacent
+// pair, and allocation in the order of the reference's recursion (a node's
+// pair, then the left subtree, then the right).
+//
+// Each node's decision depends only on its own range of `tris`, which only
+// its ancestors' partitions rearranged, so nodes may be split in any order
+// that puts parents first. The parallel build uses that: the largest nodes
+// are split first (their passes over the triangles cut into chunks, one per
+// thread), the remaining subtrees are built concurrently into local arrays,
+// and a final walk in the reference's allocation order numbers every node.
+// Per-chunk bounds and bins are folded in chunk order with the same fmin/fmax
+// calls as the one-thread fold, so the nodes come out bit-identical.
+class BvhBuilder {
+public:
+    BvhBuilder(std::vector<BvhTri>& tris, unsigned threads, size_t grain)
+        : tris_(tris), threads_(threads ? threads : 1u), grain_(grain) {}
+
+    void build(std::vector<PtBvhNode>& out);
+
+private:
+    // one node's decision: a leaf, or a split at `mid` after the partition
+    struct Decision {
+        bool leaf;
+        size_t mid;
+    };
+    struct Task {
+        size_t start, end;
+        Aabb box;
+        std::vector<PtBvhNode> nodes;  // local numbering, root at 0
+        std::vector<uint8_t> inner;    // 1 where left_first is a local node index
+    };
+    // a node of the top levels, split before the subtrees are handed out
+    struct Top {
+        size_t start, end;
+        Aabb box;
+        int kind;      // 0 leaf, 1 split (children at `left`, `left` + 1), 2 subtree task
+        size_t left;   // index into top_ (kind 1) or tasks_ (kind 2)
     };
 
-    // world.rs:1430-1591, iterative over an explicit stack that visits the
-    // left child's subtree before the right one, as the recursion does, so
-    // nodes are allocated in the same order
-    void subdivide_all(size_t total) {
-        std::vector<Item> stack;
-        stack.push_back({0, 0, total});
-        while (!stack.empty()) {
-            Item it = stack.back();
-            stack.pop_back();
-            Item left{}, right{};
-            if (!subdivide(it.node, it.start, it.end, left, right)) continue;
-            stack.push_back(right);
-            stack.push_back(left);
-        }
+    Aabb bounds(size_t start, size_t end, unsigned threads) const;
+    Decision decide(const Aabb& box, size_t start, size_t end, unsigned threads);
+    template <class GetBin>
+    void partition_parallel(size_t start, size_t cut, size_t end, unsigned threads, const GetBin& get_bin, int axis,
+                            size_t split_bin);
+    void build_task(Task& t);
+    unsigned chunks_for(size_t count) const {
+        if (threads_ <= 1 || count < 2 * grain_) return 1u;
+        return (unsigned)std::min<size_t>(threads_, count / grain_);
     }
 
-    // split one node; returns false if it became a leaf
-    bool subdivide(size_t ni, size_t start, size_t end, Item& left_out, Item& right_out) {
-        const size_t tri_count = end - start;
-        if (tri_count <= 2) {
-            nodes[ni].left_first = (uint32_t)start;
-            nodes[ni].tri_count = (uint32_t)tri_count;
-            return false;
-        }
-        float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (size_t i = start; i < end; ++i) {
+    std::vector<BvhTri>& tris_;
+    unsigned threads_;
+    size_t grain_;
+    std::vector<Top> top_;
+    std::vector<Task> tasks_;
+};
+
+// union of the triangle boxes over [start, end), in index order
+Aabb BvhBuilder::bounds(size_t start, size_t end, unsigned threads) const {
+    const size_t n = end - start;
+    if (threads <= 1) {
+        Aabb b;
+        for (size_t i = start; i < end; ++i) b.grow(tris_[i].box);
+        return b;
+    }
+    std::vector<Aabb> part(threads);
+    run_parallel(threads, [&](unsigned t) {
+        const size_t a = start + n * t / threads, e = start + n * (t + 1) / threads;
+        Aabb b;
+        for (size_t i = a; i < e; ++i) b.grow(tris_[i].box);
+        part[t] = b;
+    });
+    Aabb b;
+    for (const Aabb& p : part) b.grow(p);
+    return b;
+}
+
+BvhBuilder::Decision BvhBuilder::decide(const Aabb& node_box, size_t start, size_t end, unsigned threads) {
+    const size_t tri_count = end - start;
+    if (tri_count <= 2) return {true, 0};
+    const size_t n = tri_count;
+    // centroid bounds
+    struct Bounds {
+        float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    };
+    // per-chunk partials; the one-chunk case (every node below the top levels) stays on the stack
+    Bounds c_one;
+    std::vector<Bounds> c_many(threads > 1 ? threads : 0);
+    Bounds* cpart = threads > 1 ? c_many.data() : &c_one;
+    run_parallel(threads, [&](unsigned t) {
+        const size_t a = start + n * t / threads, e = start + n * (t + 1) / threads;
+        Bounds c;
+        for (size_t i = a; i < e; ++i)
             for (int k = 0; k < 3; ++k) {
-                cmin[k] = std::fmin(cmin[k], tris[i].centroid[k]);
-                cmax[k] = std::fmax(cmax[k], tris[i].centroid[k]);
+                c.mn[k] = min_f(c.mn[k], tris_[i].centroid[k]);
+                c.mx[k] = max_f(c.mx[k], tris_[i].centroid[k]);
+            }
+        cpart[t] = c;
+    });
+    float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (unsigned t = 0; t < threads; ++t)
+        for (int k = 0; k < 3; ++k) {
+            cmin[k] = min_f(cmin[k], cpart[t].mn[k]);
+            cmax[k] = max_f(cmax[k], cpart[t].mx[k]);
+        }
+    // bins of the three axes in one pass; an axis with a flat centroid range is skipped
+    bool live[3];
+    float scale[3];
+    for (int k = 0; k < 3; ++k) {
+        live[k] = !(cmin[k] == cmax[k]);
+        scale[k] = (float)PT_BVH_BINS / (cmax[k] - cmin[k]);
+    }
+    struct Bins {
+        Aabb box[3][PT_BVH_BINS];
+        uint32_t count[3][PT_BVH_BINS] = {};
+    };
+    Bins b_one;
+    std::vector<Bins> b_many(threads > 1 ? threads : 0);
+    Bins* bpart = threads > 1 ? b_many.data() : &b_one;
+    run_parallel(threads, [&](unsigned t) {
+        const size_t a = start + n * t / threads, e = start + n * (t + 1) / threads;
+        Bins& bn = bpart[t];
+        for (size_t i = a; i < e; ++i) {
+            const BvhTri& tri = tris_[i];
+            for (int k = 0; k < 3; ++k) {
+                if (!live[k]) continue;
+                size_t b = std::min(f32_as_usize((tri.centroid[k] - cmin[k]) * scale[k]), (size_t)PT_BVH_BINS - 1);
+                bn.count[k][b] += 1;
+                bn.box[k][b].grow(tri.box);
             }
         }
-        int best_axis = 0;
-        size_t best_split = 0;
-        float best_cost = std::numeric_limits<float>::max();
-        for (int axis = 0; axis < 3; ++axis) {
-            const float amin = cmin[axis], amax = cmax[axis];
-            if (amin == amax) continue;
-            Aabb bins[PT_BVH_BINS];
-            uint32_t counts[PT_BVH_BINS] = {0};
-            const float scale = (float)PT_BVH_BINS / (amax - amin);
-            for (size_t i = start; i < end; ++i) {
-                size_t b = std::min(f32_as_usize((tris[i].centroid[axis] - amin) * scale), (size_t)PT_BVH_BINS - 1);
-                counts[b] += 1;
-                bins[b].grow(tris[i].box);
+    });
+    int best_axis = 0;
+    size_t best_split = 0, best_left = 0;
+    float best_cost = std::numeric_limits<float>::max();
+    for (int axis = 0; axis < 3; ++axis) {
+        if (!live[axis]) continue;
+        Aabb bins[PT_BVH_BINS];
+        uint32_t counts[PT_BVH_BINS] = {0};
+        for (unsigned t = 0; t < threads; ++t)
+            for (int b = 0; b < PT_BVH_BINS; ++b) {
+                counts[b] += bpart[t].count[axis][b];
+                bins[b].grow(bpart[t].box[axis][b]);
             }
-            float left_area[PT_BVH_BINS - 1], right_area[PT_BVH_BINS - 1];
-            uint32_t left_count[PT_BVH_BINS - 1], right_count[PT_BVH_BINS - 1];
-            Aabb lb;
-            uint32_t ls = 0;
-            for (int i = 0; i < PT_BVH_BINS - 1; ++i) {
-                ls += counts[i];
-                lb.grow(bins[i]);
-                left_count[i] = ls;
-                left_area[i] = half_area(lb.mn, lb.mx);
-            }
-            Aabb rb;
-            uint32_t rs = 0;
-            for (int i = PT_BVH_BINS - 1; i >= 1; --i) {
-                rs += counts[i];
-                rb.grow(bins[i]);
-                right_count[i - 1] = rs;
-                right_area[i - 1] = half_area(rb.mn, rb.mx);
-            }
-            for (int i = 0; i < PT_BVH_BINS - 1; ++i) {
-                float cost = std::fmaf((float)left_count[i], left_area[i], (float)right_count[i] * right_area[i]);
-                if (cost < best_cost) {
-                    best_cost = cost;
-                    best_axis = axis;
-                    best_split = (size_t)i;
-                }
+        float left_area[PT_BVH_BINS - 1], right_area[PT_BVH_BINS - 1];
+        uint32_t left_count[PT_BVH_BINS - 1], right_count[PT_BVH_BINS - 1];
+        Aabb lb;
+        uint32_t ls = 0;
+        for (int i = 0; i < PT_BVH_BINS - 1; ++i) {
+            ls += counts[i];
+            lb.grow(bins[i]);
+            left_count[i] = ls;
+            left_area[i] = half_area(lb.mn, lb.mx);
+        }
+        Aabb rb;
+        uint32_t rs = 0;
+        for (int i = PT_BVH_BINS - 1; i >= 1; --i) {
+            rs += counts[i];
+            rb.grow(bins[i]);
+            right_count[i - 1] = rs;
+            right_area[i - 1] = half_area(rb.mn, rb.mx);
+        }
+        for (int i = 0; i < PT_BVH_BINS - 1; ++i) {
+            float cost = std::fmaf((float)left_count[i], left_area[i], (float)right_count[i] * right_area[i]);
+            if (cost < best_cost) {
+                best_cost = cost;
+                best_axis = axis;
+                best_split = (size_t)i;
+                best_left = left_count[i];
             }
         }
-        const float leaf_cost = (float)tri_count * half_area(nodes[ni].aabb_min, nodes[ni].aabb_max);
-        if (best_cost >= leaf_cost) {
-            nodes[ni].left_first = (uint32_t)start;
-            nodes[ni].tri_count = (uint32_t)tri_count;
-            return false;
-        }
-        const float amin = cmin[best_axis], amax = cmax[best_axis];
-        const float scale = (float)PT_BVH_BINS / (amax - amin);
-        auto get_bin = [&](float c) { return std::min(f32_as_usize((c - amin) * scale), (size_t)PT_BVH_BINS - 1); };
+    }
+    const float leaf_cost = (float)tri_count * half_area(node_box.mn, node_box.mx);
+    if (best_cost >= leaf_cost) return {true, 0};
+    const float amin = cmin[best_axis];
+    const float sc = scale[best_axis];
+    auto get_bin = [&](float c) { return std::min(f32_as_usize((c - amin) * sc), (size_t)PT_BVH_BINS - 1); };
+    size_t mid;
+    if (threads <= 1) {
         // modified Hoare partition (world.rs:1554-1573)
         int64_t left = (int64_t)start, right = (int64_t)end - 1;
         for (;;) {
-            while (left <= right && get_bin(tris[left].centroid[best_axis]) <= best_split) ++left;
-            while (right > left && get_bin(tris[right].centroid[best_axis]) > best_split) --right;
+            while (left <= right && get_bin(tris_[left].centroid[best_axis]) <= best_split) ++left;
+            while (right > left && get_bin(tris_[right].centroid[best_axis]) > best_split) --right;
             if (left >= right) break;
-            std::swap(tris[left], tris[right]);
+            std::swap(tris_[left], tris_[right]);
             ++left;
             --right;
         }
-        const size_t split = (size_t)left;
-        const size_t lc = nodes.size(), rc = lc + 1;
-        nodes.push_back(PtBvhNode{});
-        nodes.push_back(PtBvhNode{});
-        set_box(lc, start, split);
-        set_box(rc, split, end);
-        nodes[ni].left_first = (uint32_t)lc;
-        nodes[ni].tri_count = 0;
-        left_out = {lc, start, split};
-        right_out = {rc, split, end};
-        return true;
+        mid = (size_t)left;
+    } else {
+        mid = start + best_left;
+        partition_parallel(start, mid, end, threads, get_bin, best_axis, best_split);
     }
-};
+    if (mid == start || mid == end) throw BvhNoProgress{};
+    return {false, mid};
+}
+
+// The Hoare partition above swaps the k-th triangle of [start, cut) that
+// belongs right (in ascending order) with the k-th triangle of [cut, end) that
+// belongs left (in descending order), for every k, where cut - start is the
+// count of triangles that belong left. The pairs are disjoint, so they are
+// found by chunked scans and swapped concurrently, with the same result.
+template <class GetBin>
+void BvhBuilder::partition_parallel(size_t start, size_t cut, size_t end, unsigned threads, const GetBin& get_bin,
+                                    int axis, size_t split_bin) {
+    const size_t nl = cut - start, nr = end - cut;
+    auto goes_right = [&](size_t i) { return get_bin(tris_[i].centroid[axis]) > split_bin; };
+    // chunk t of the left part is [start + nl*t/T, start + nl*(t+1)/T); chunk t
+    // of the right part is the t-th slice counted down from end
+    auto lchunk = [&](unsigned t, size_t& a, size_t& e) {
+        a = start + nl * t / threads;
+        e = start + nl * (t + 1) / threads;
+    };
+    auto rchunk = [&](unsigned t, size_t& hi, size_t& lo) {
+        hi = end - nr * t / threads;
+        lo = end - nr * (t + 1) / threads;
+    };
+    std::vector<size_t> lcount(threads + 1, 0), rcount(threads + 1, 0);
+    run_parallel(threads, [&](unsigned t) {
+        size_t a, e, hi, lo, c = 0;
+        lchunk(t, a, e);
+        for (size_t i = a; i < e; ++i) c += goes_right(i);
+        lcount[t + 1] = c;
+        c = 0;
+        rchunk(t, hi, lo);
+        for (size_t i = lo; i < hi; ++i) c += !goes_right(i);
+        rcount[t + 1] = c;
+    });
+    for (unsigned t = 0; t < threads; ++t) {
+        lcount[t + 1] += lcount[t];
+        rcount[t + 1] += rcount[t];
+    }
+    const size_t swaps = lcount[threads];
+    if (swaps != rcount[threads]) throw std::logic_error("BVH partition counts disagree with the bins");
+    std::vector<size_t> lpos(swaps), rpos(swaps);
+    run_parallel(threads, [&](unsigned t) {
+        size_t a, e, hi, lo, k = lcount[t];
+        lchunk(t, a, e);
+        for (size_t i = a; i < e; ++i)
+            if (goes_right(i)) lpos[k++] = i;
+        k = rcount[t];
+        rchunk(t, hi, lo);
+        for (size_t i = hi; i-- > lo;)
+            if (!goes_right(i)) rpos[k++] = i;
+    });
+    run_parallel(threads, [&](unsigned t) {
+        const size_t a = swaps * t / threads, e = swaps * (t + 1) / threads;
+        for (size_t k = a; k < e; ++k) std::swap(tris_[lpos[k]], tris_[rpos[k]]);
+    });
+}
+
+// one subtree, depth first in the reference's order, into the task's local arrays
+void BvhBuilder::build_task(Task& t) {
+    struct Item {
+        size_t node, start, end;
+    };
+    auto set_box = [](PtBvhNode& nd, const Aabb& b) {
+        for (int k = 0; k < 3; ++k) {
+            nd.aabb_min[k] = b.mn[k];
+            nd.aabb_max[k] = b.mx[k];
+        }
+    };
+    t.nodes.clear();
+    t.inner.clear();
+    t.nodes.reserve(2 * (t.end - t.start));
+    t.inner.reserve(2 * (t.end - t.start));
+    t.nodes.push_back(PtBvhNode{});
+    t.inner.push_back(0);
+    set_box(t.nodes[0], t.box);
+    std::vector<Item> stack;
+    stack.push_back({0, t.start, t.end});
+    while (!stack.empty()) {
+        Item it = stack.back();
+        stack.pop_back();
+        Aabb box;
+        for (int k = 0; k < 3; ++k) {
+            box.mn[k] = t.nodes[it.node].aabb_min[k];
+            box.mx[k] = t.nodes[it.node].aabb_max[k];
+        }
+        Decision d = decide(box, it.start, it.end, 1u);
+        if (d.leaf) {
+            t.nodes[it.node].left_first = (uint32_t)it.start;
+            t.nodes[it.node].tri_count = (uint32_t)(it.end - it.start);
+            continue;
+        }
+        const size_t lc = t.nodes.size();
+        t.nodes.push_back(PtBvhNode{});
+        t.nodes.push_back(PtBvhNode{});
+        t.inner.push_back(0);
+        t.inner.push_back(0);
+        set_box(t.nodes[lc], bounds(it.start, d.mid, 1u));
+        set_box(t.nodes[lc + 1], bounds(d.mid, it.end, 1u));
+        t.nodes[it.node].left_first = (uint32_t)lc;
+        t.nodes[it.node].tri_count = 0;
+        t.inner[it.node] = 1;
+        stack.push_back({lc + 1, d.mid, it.end});
+        stack.push_back({lc, it.start, d.mid});
+    }
+}
+
+void BvhBuilder::build(std::vector<PtBvhNode>& out) {
+    const size_t n = tris_.size();
+    top_.clear();
+    tasks_.clear();
+    top_.push_back(Top{0, n, bounds(0, n, chunks_for(n)), 2, 0});
+    // split the largest pending node while it is worth more than one task's share
+    const size_t target_tasks = threads_ <= 1 ? 1 : 8 * (size_t)threads_;
+    auto larger = [&](size_t a, size_t b) { return top_[a].end - top_[a].start < top_[b].end - top_[b].start; };
+    std::vector<size_t> pending{0};
+    while (!pending.empty()) {
+        std::pop_heap(pending.begin(), pending.end(), larger);
+        const size_t x = pending.back();
+        const size_t count = top_[x].end - top_[x].start;
+        if (threads_ <= 1 || count < 2 * grain_ || pending.size() >= target_tasks) break;
+        pending.pop_back();
+        const Decision d = decide(top_[x].box, top_[x].start, top_[x].end, chunks_for(count));
+        if (d.leaf) {
+            top_[x].kind = 0;
+            continue;
+        }
+        const size_t s = top_[x].start, e = top_[x].end;
+        Top l{s, d.mid, bounds(s, d.mid, chunks_for(d.mid - s)), 2, 0};
+        Top r{d.mid, e, bounds(d.mid, e, chunks_for(e - d.mid)), 2, 0};
+        top_[x].kind = 1;
+        top_[x].left = top_.size();
+        top_.push_back(l);
+        top_.push_back(r);
+        pending.push_back(top_[x].left);
+        std::push_heap(pending.begin(), pending.end(), larger);
+        pending.push_back(top_[x].left + 1);
+        std::push_heap(pending.begin(), pending.end(), larger);
+    }
+    // the rest are subtree tasks, handed out largest first
+    std::sort(pending.begin(), pending.end(), [&](size_t a, size_t b) { return larger(b, a); });
+    tasks_.resize(pending.size());
+    for (size_t i = 0; i < pending.size(); ++i) {
+        Top& tp = top_[pending[i]];
+        tasks_[i].start = tp.start;
+        tasks_[i].end = tp.end;
+        tasks_[i].box = tp.box;
+        tp.left = i;
+    }
+    std::atomic<size_t> next_task{0};
+    run_parallel(std::min<size_t>(threads_, tasks_.size()), [&](unsigned) {
+        for (size_t i; (i = next_task.fetch_add(1)) < tasks_.size();) build_task(tasks_[i]);
+    });
+    // number the nodes in the reference's allocation order: a split node takes
+    // the next pair, a subtree takes its local count minus its root
+    size_t total = 1;
+    for (const Top& tp : top_)
+        if (tp.kind == 1) total += 2;
+    for (const Task& t : tasks_) total += t.nodes.size() - 1;
+    if (total > 0xFFFFFFFFull) throw std::bad_alloc();
+    out.assign(total, PtBvhNode{});
+    std::vector<size_t> task_root(tasks_.size()), task_base(tasks_.size());
+    struct Item {
+        size_t top, global;
+    };
+    std::vector<Item> stack{{0, 0}};
+    size_t next = 1;
+    while (!stack.empty()) {
+        Item it = stack.back();
+        stack.pop_back();
+        const Top& tp = top_[it.top];
+        PtBvhNode& nd = out[it.global];
+        for (int k = 0; k < 3; ++k) {
+            nd.aabb_min[k] = tp.box.mn[k];
+            nd.aabb_max[k] = tp.box.mx[k];
+        }
+        if (tp.kind == 0) {
+            nd.left_first = (uint32_t)tp.start;
+            nd.tri_count = (uint32_t)(tp.end - tp.start);
+        } else if (tp.kind == 1) {
+            nd.left_first = (uint32_t)next;
+            nd.tri_count = 0;
+            stack.push_back({tp.left + 1, next + 1});
+            stack.push_back({tp.left, next});
+            next += 2;
+        } else {
+            task_root[tp.left] = it.global;
+            task_base[tp.left] = next;
+            next += tasks_[tp.left].nodes.size() - 1;
+        }
+    }
+    // copy each subtree into its block, local index i >= 1 at base + i - 1
+    std::atomic<size_t> next_copy{0};
+    run_parallel(std::min<size_t>(threads_, tasks_.size()), [&](unsigned) {
+        for (size_t i; (i = next_copy.fetch_add(1)) < tasks_.size();) {
+            const Task& t = tasks_[i];
+            const size_t base = task_base[i];
+            auto place = [&](size_t local) { return local == 0 ? task_root[i] : base + local - 1; };
+            for (size_t j = 0; j < t.nodes.size(); ++j) {
+                PtBvhNode nd = t.nodes[j];
+                if (t.inner[j]) nd.left_first = (uint32_t)place(nd.left_first);
+                out[place(j)] = nd;
+            }
+        }
+    });
+}
 
 // luminance of max(rgb, 0) in f64 (world.rs:578-581)
 inline double luminance(const float* rgb) {
@@ -417,28 +770,34 @@ pt_status pt_world_build(const pt_world_desc* d, pt_world** out) {
     try {
         const uint32_t n = d->triangle_count;
         // BVH triangles in input order (world.rs:1290-1297)
+        const unsigned threads = host_threads();
+        const size_t grain = bvh_task_grain();
+        // element-wise passes: one chunk per thread once there is enough work
+        const unsigned pass_threads = (unsigned)std::max<size_t>(1, std::min<size_t>(threads, n / (4 * grain)));
+        auto each = [&](auto&& f) {
+            run_parallel(pass_threads, [&](unsigned t) {
+                const size_t a = (size_t)n * t / pass_threads, e = (size_t)n * (t + 1) / pass_threads;
+                for (size_t i = a; i < e; ++i) f(i);
+            });
+        };
         std::vector<BvhTri> tris(n);
-        for (uint32_t i = 0; i < n; ++i) {
+        each([&](size_t i) {
             const float* v = d->geometries[i].v;
             BvhTri& t = tris[i];
             for (int k = 0; k < 3; ++k) {
-                t.box.mn[k] = std::fmin(std::fmin(v[k], v[3 + k]), v[6 + k]);
-                t.box.mx[k] = std::fmax(std::fmax(v[k], v[3 + k]), v[6 + k]);
+                t.box.mn[k] = min_f(min_f(v[k], v[3 + k]), v[6 + k]);
+                t.box.mx[k] = max_f(max_f(v[k], v[3 + k]), v[6 + k]);
                 t.centroid[k] = ((v[k] + v[3 + k]) + v[6 + k]) / 3.0f;
             }
-            t.original = i;
-        }
-        w->bvh_nodes.reserve(2 * (size_t)n);
-        w->bvh_nodes.push_back(PtBvhNode{});
-        Builder b{w->bvh_nodes, tris};
-        b.set_box(0, 0, n);
-        b.subdivide_all(n);
+            t.original = (uint32_t)i;
+        });
+        BvhBuilder(tris, threads, grain).build(w->bvh_nodes);
         w->geometries.resize(n);
         w->attributes.resize(n);
-        for (uint32_t i = 0; i < n; ++i) {
+        each([&](size_t i) {
             w->geometries[i] = d->geometries[tris[i].original];
             w->attributes[i] = d->attributes[tris[i].original];
-        }
+        });
         w->materials.assign(d->materials, d->materials + d->material_count);
         w->atlas_size = d->atlas_size;
         const size_t layer_bytes = (size_t)d->atlas_size * d->atlas_size * 4;
@@ -524,6 +883,17 @@ pt_status pt_world_build(const pt_world_desc* d, pt_world** out) {
     } catch (const std::bad_alloc&) {
         delete w;
         return set_error(PT_ERR_MEMORY, "out of host memory while building the world");
+    } catch (const BvhNoProgress&) {
+        delete w;
+        return set_error(PT_ERR_INVALID,
+                         "BVH split made no progress: vertex positions are non-finite or too large for the "
+                         "surface-area costs");
+    } catch (const std::system_error& e) {
+        delete w;
+        return set_error(PT_ERR_MEMORY, "could not start host threads: %s", e.what());
+    } catch (const std::exception& e) {
+        delete w;
+        return set_error(PT_ERR_INVALID, "world build failed: %s", e.what());
     }
     *out = w;
     return PT_OK;

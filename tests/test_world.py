@@ -1,16 +1,42 @@
 """Host world preparation (C++ in libptcore) against the Python restatement
 of the reference's producers (oracle/world_oracle.py), bit for bit, plus the
-BVH invariants the reference relies on. CPU only."""
+BVH invariants the reference relies on. CPU only.
+
+The BVH build runs on several threads for large scenes. PT_HOST_THREADS and
+PT_BVH_TASK_GRAIN force that path onto small scenes here, so the oracle
+comparison covers both the one-thread and the threaded builds."""
+import contextlib
+import os
+import subprocess
+import sys
+
 import numpy as np
 import pytest
 
 import ptcore
 import world_oracle as wo
 
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
-def generated(kind, seed, w, h, tris=0):
+
+@contextlib.contextmanager
+def build_env(**env):
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update({k: str(v) for k, v in env.items()})
+    try:
+        yield
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def generated(kind, seed, w, h, tris=0, **env):
     g = ptcore.Generated(kind, seed, w, h, tris)
-    world = ptcore.World(g.desc, keep=g)
+    with build_env(**env):
+        world = ptcore.World(g.desc, keep=g)
     return g, world
 
 
@@ -26,10 +52,14 @@ def oracle_world(g):
                             d.skybox_width, d.skybox_height, d.skybox_luminance, cam)
 
 
-@pytest.fixture(scope="module", params=[("cornell", ptcore.SCENE_CORNELL, 0), ("sized2k", ptcore.SCENE_SIZED, 2000)])
+@pytest.fixture(scope="module", params=[("cornell", ptcore.SCENE_CORNELL, 0, {}),
+                                        ("sized2k", ptcore.SCENE_SIZED, 2000, {"PT_HOST_THREADS": 1}),
+                                        ("sized2k_threaded", ptcore.SCENE_SIZED, 2000,
+                                         {"PT_HOST_THREADS": 4, "PT_BVH_TASK_GRAIN": 16})],
+                ids=lambda p: p[0])
 def pair(request):
-    name, kind, tris = request.param
-    g, world = generated(kind, 11, 320, 240, tris)
+    name, kind, tris, env = request.param
+    g, world = generated(kind, 11, 320, 240, tris, **env)
     return name, g, world, oracle_world(g)
 
 
@@ -153,3 +183,57 @@ def test_world_rejects_bad_input():
     a["attributes"][3]["material"] = 999
     with pytest.raises(ptcore.PtError):
         ptcore.World(d)
+
+
+def test_threaded_bvh_build_is_identical():
+    """The threaded build splits the top levels with chunked passes and a
+    parallel partition, then builds subtrees concurrently: nodes and the
+    triangle order must not depend on the thread count or the task grain."""
+    g = ptcore.Generated(ptcore.SCENE_SIZED, 5, 640, 480, 120000)
+    digests = set()
+    for env in ({"PT_HOST_THREADS": 1}, {"PT_HOST_THREADS": 8}, {"PT_HOST_THREADS": 3, "PT_BVH_TASK_GRAIN": 7},
+                {"PT_HOST_THREADS": 16, "PT_BVH_TASK_GRAIN": 1}):
+        with build_env(**env):
+            w = ptcore.World(g.desc, keep=g)
+        a = w.arrays()
+        digests.add(a["bvh_nodes"].tobytes() + a["geometries"].tobytes() + a["attributes"].tobytes())
+    assert len(digests) == 1
+
+
+NO_PROGRESS_CHILD = r'''
+import resource, sys
+resource.setrlimit(resource.RLIMIT_AS, (6 << 30, 6 << 30))
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+import ptcore
+g = ptcore.Generated(ptcore.SCENE_CORNELL, 1, 64, 64)
+a = g.arrays()
+for case in ("huge", "nan"):
+    geo = np.zeros(3, ptcore.GEOMETRY_DTYPE)
+    v = geo["v"].reshape(3, 3, 3)
+    if case == "huge":
+        # x spans -3e38..3e38 (every area is infinite) around the same centroid x
+        for i in range(3):
+            v[i] = [[-3e38, i, 0.0], [3e38, i + 0.5, 0.0], [0.0, i, 1.0]]
+    else:
+        v[:] = np.nan
+    att = np.zeros(3, ptcore.ATTRIBUTE_DTYPE)
+    inp = ptcore.WorldInput(geo, att, a["materials"][:1], a["skybox"])
+    try:
+        inp.build()
+        print(case, "built")
+    except ptcore.PtError as e:
+        print(case, "error", e.code, str(e))
+'''
+
+
+def test_bvh_without_progress_is_an_error():
+    """Boxes whose surface-area costs are all non-finite make the reference's
+    recursion repeat a node forever. The builder reports an error instead."""
+    r = subprocess.run([sys.executable, "-c", NO_PROGRESS_CHILD, os.path.join(ROOT, "path-tracer_amd")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = dict(line.split(" ", 1) for line in r.stdout.strip().splitlines())
+    for case in ("huge", "nan"):
+        assert lines[case].startswith(f"error {ptcore.PT_ERR_INVALID} "), lines
+        assert "made no progress" in lines[case]
