@@ -26,9 +26,10 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
-#include <unordered_map>
+#include <limits>
 #include <vector>
 
+#include "host_parallel.h"
 #include "tangents.h"
 
 namespace ptcore {
@@ -113,26 +114,73 @@ bool generate_tangents(const MeshView& mv, std::vector<float>& out) {
     //    NaN never compares equal, so such corners stay unique
     std::vector<int> list((size_t)nf * 3);
     {
-        std::unordered_map<std::array<uint32_t, 8>, int, KeyHash> seen;
-        seen.reserve((size_t)nf * 3);
-        for (int f = 0; f < nf; ++f)
-            for (int i = 0; i < 3; ++i) {
-                const int idx = 4 * f + i;
-                V p = M.pos(idx), n = M.nrm(idx), t = M.tex(idx);
-                float vals[8] = {p.x, p.y, p.z, n.x, n.y, n.z, t.x, t.y};
-                bool has_nan = false;
-                std::array<uint32_t, 8> k;
+        // Corners that share a vertex index share their data, so the weld
+        // works per vertex: each vertex's first corner in face order, its
+        // key, and then, over the vertices with equal keys, the least first
+        // corner, which is the representative. A vertex with a NaN never
+        // welds. Keys are split among the threads by hash, so each key is
+        // resolved by one thread, and a least-corner rule does not depend on
+        // the order or the thread count.
+        const size_t nc = (size_t)nf * 3;
+        uint32_t nv = 0;
+        for (size_t j = 0; j < nc; ++j) nv = std::max(nv, mv.indices[j] + 1u);
+        constexpr int NONE = std::numeric_limits<int>::max();
+        std::vector<int> first(nv, NONE);  // vertex -> first corner slot (3 * face + corner)
+        for (size_t slot = nc; slot-- > 0;) first[mv.indices[slot]] = (int)slot;
+        std::vector<std::array<uint32_t, 8>> keys(nv);
+        std::vector<uint64_t> hashes(nv);
+        std::vector<uint8_t> has_nan(nv, 0);
+        const unsigned threads = nv >= 65536 ? host_threads() : 1u;
+        const KeyHash hash;
+        run_slices(nv, threads, [&](size_t a, size_t e) {
+            for (size_t v = a; v < e; ++v) {
+                if (first[v] == NONE) continue;
+                const float* p = &mv.positions[3 * v];
+                const float* n = &mv.normals[3 * v];
+                const float* t = &mv.texcoords[2 * v];
+                const float vals[8] = {p[0], p[1], p[2], n[0], n[1], n[2], t[0], t[1]};
                 for (int c = 0; c < 8; ++c) {
-                    has_nan = has_nan || std::isnan(vals[c]);
-                    k[c] = key_bits(vals[c]);
+                    has_nan[v] |= std::isnan(vals[c]) ? 1 : 0;
+                    keys[v][c] = key_bits(vals[c]);
                 }
-                if (has_nan) {
-                    list[3 * (size_t)f + i] = idx;
-                    continue;
-                }
-                auto it = seen.emplace(k, idx).first;
-                list[3 * (size_t)f + i] = it->second;
+                hashes[v] = hash(keys[v]);
             }
+        });
+        std::vector<int> rep(nv, NONE);  // vertex -> least first corner over its key
+        run_parallel(threads, [&](unsigned part) {
+            auto mine = [&](size_t v) {
+                return first[v] != NONE && !has_nan[v] && (uint32_t)hashes[v] % threads == part;
+            };
+            size_t count = 0;
+            for (size_t v = 0; v < nv; ++v) count += mine(v);
+            size_t cap = 16;
+            while (cap < 2 * count) cap <<= 1;
+            std::vector<int> owner(cap, -1), least(cap);
+            auto find = [&](size_t v) {
+                size_t h = (size_t)(hashes[v] >> 32) & (cap - 1);
+                while (owner[h] >= 0 && keys[(size_t)owner[h]] != keys[v]) h = (h + 1) & (cap - 1);
+                return h;
+            };
+            for (size_t v = 0; v < nv; ++v) {
+                if (!mine(v)) continue;
+                const size_t h = find(v);
+                if (owner[h] < 0) {
+                    owner[h] = (int)v;
+                    least[h] = first[v];
+                } else {
+                    least[h] = std::min(least[h], first[v]);
+                }
+            }
+            for (size_t v = 0; v < nv; ++v)
+                if (mine(v)) rep[v] = least[find(v)];
+        });
+        auto corner_idx = [](size_t slot) { return (int)(4 * (slot / 3) + slot % 3); };
+        run_slices(nc, threads, [&](size_t a, size_t e) {
+            for (size_t slot = a; slot < e; ++slot) {
+                const uint32_t v = mv.indices[slot];
+                list[slot] = has_nan[v] ? corner_idx(slot) : corner_idx((size_t)rep[v]);
+            }
+        });
     }
     // 2. degenerate triangles (equal corner positions) go to the back; good ones keep their order
     std::vector<Tri> tris(nf);
@@ -154,28 +202,30 @@ bool generate_tangents(const MeshView& mv, std::vector<float>& out) {
         tris[t].flag = t >= ngood ? MARK_DEGENERATE : 0;
     }
     // 3. triangle derivative frames
-    for (int t = 0; t < ngood; ++t) {
-        Tri& T = tris[t];
-        T.flag |= GROUP_WITH_ANY;  // assumed bad until shown good
-        const V v1 = M.pos(tl[3 * t]), v2 = M.pos(tl[3 * t + 1]), v3 = M.pos(tl[3 * t + 2]);
-        const V t1 = M.tex(tl[3 * t]), t2 = M.tex(tl[3 * t + 1]), t3 = M.tex(tl[3 * t + 2]);
-        const float t21x = t2.x - t1.x, t21y = t2.y - t1.y, t31x = t3.x - t1.x, t31y = t3.y - t1.y;
-        const V d1 = vsub(v2, v1), d2 = vsub(v3, v1);
-        const float area2 = t21x * t31y - t21y * t31x;
-        const V os = vsub(vscale(t31y, d1), vscale(t21y, d2));
-        const V ot = vadd(vscale(-t31x, d1), vscale(t21x, d2));
-        if (area2 > 0) T.flag |= ORIENT_PRESERVING;
-        if (nz(area2)) {
-            const float abs_area = std::fabs(area2);
-            const float len_os = vlen(os), len_ot = vlen(ot);
-            const float s = (T.flag & ORIENT_PRESERVING) ? 1.0f : -1.0f;
-            if (nz(len_os)) T.os = vscale(s / len_os, os);
-            if (nz(len_ot)) T.ot = vscale(s / len_ot, ot);
-            T.mag_s = len_os / abs_area;
-            T.mag_t = len_ot / abs_area;
-            if (nz(T.mag_s) && nz(T.mag_t)) T.flag &= ~GROUP_WITH_ANY;
+    run_slices((size_t)ngood, ngood >= 65536 ? host_threads() : 1u, [&](size_t ta, size_t te) {
+        for (int t = (int)ta; t < (int)te; ++t) {
+            Tri& T = tris[t];
+            T.flag |= GROUP_WITH_ANY;  // assumed bad until shown good
+            const V v1 = M.pos(tl[3 * t]), v2 = M.pos(tl[3 * t + 1]), v3 = M.pos(tl[3 * t + 2]);
+            const V t1 = M.tex(tl[3 * t]), t2 = M.tex(tl[3 * t + 1]), t3 = M.tex(tl[3 * t + 2]);
+            const float t21x = t2.x - t1.x, t21y = t2.y - t1.y, t31x = t3.x - t1.x, t31y = t3.y - t1.y;
+            const V d1 = vsub(v2, v1), d2 = vsub(v3, v1);
+            const float area2 = t21x * t31y - t21y * t31x;
+            const V os = vsub(vscale(t31y, d1), vscale(t21y, d2));
+            const V ot = vadd(vscale(-t31x, d1), vscale(t21x, d2));
+            if (area2 > 0) T.flag |= ORIENT_PRESERVING;
+            if (nz(area2)) {
+                const float abs_area = std::fabs(area2);
+                const float len_os = vlen(os), len_ot = vlen(ot);
+                const float s = (T.flag & ORIENT_PRESERVING) ? 1.0f : -1.0f;
+                if (nz(len_os)) T.os = vscale(s / len_os, os);
+                if (nz(len_ot)) T.ot = vscale(s / len_ot, ot);
+                T.mag_s = len_os / abs_area;
+                T.mag_t = len_ot / abs_area;
+                if (nz(T.mag_s) && nz(T.mag_t)) T.flag &= ~GROUP_WITH_ANY;
+            }
         }
-    }
+    });
     // 4. neighbours: edges keyed (min, max, triangle), fully sorted
     struct Edge {
         int i0, i1, f;
@@ -298,26 +348,46 @@ bool generate_tangents(const MeshView& mv, std::vector<float>& out) {
         fr.ot = vnz(sum_t) ? vnorm(sum_t) : sum_t;
         return fr;
     };
+    // scratch reused across groups: the members' derivatives projected once
+    // into the group's tangent plane, and the distinct subgroups so far
+    std::vector<V> proj_s, proj_t;
+    std::vector<int> members;
+    std::vector<std::vector<int>> subgroups;
+    std::vector<Frame> sub_frames;
     for (const Group& G : groups) {
-        std::vector<std::vector<int>> subgroups;
-        std::vector<Frame> sub_frames;
         const V n = M.nrm(G.rep);
-        for (int f : G.faces) {
+        const size_t k = G.faces.size();
+        proj_s.resize(k);
+        proj_t.resize(k);
+        for (size_t j = 0; j < k; ++j) {
+            proj_s[j] = project(tris[G.faces[j]].os, n);
+            proj_t[j] = project(tris[G.faces[j]].ot, n);
+        }
+        size_t nsub = 0;
+        for (size_t a = 0; a < k; ++a) {
+            const int f = G.faces[a];
             const Tri& F = tris[f];
-            const V os1 = project(F.os, n), ot1 = project(F.ot, n);
-            std::vector<int> members;
-            for (int t : G.faces) {
+            members.clear();
+            for (size_t b = 0; b < k; ++b) {
+                const int t = G.faces[b];
                 const Tri& T = tris[t];
-                const V os2 = project(T.os, n), ot2 = project(T.ot, n);
                 const bool any = ((F.flag | T.flag) & GROUP_WITH_ANY) != 0;
-                if (any || F.face == T.face || (vdot(os1, os2) > -1.0f && vdot(ot1, ot2) > -1.0f)) members.push_back(t);
+                if (any || F.face == T.face ||
+                    (vdot(proj_s[a], proj_s[b]) > -1.0f && vdot(proj_t[a], proj_t[b]) > -1.0f))
+                    members.push_back(t);
             }
             std::sort(members.begin(), members.end());
             size_t l = 0;
-            while (l < subgroups.size() && subgroups[l] != members) ++l;
-            if (l == subgroups.size()) {
-                subgroups.push_back(members);
-                sub_frames.push_back(eval_frame(members, G.rep));
+            while (l < nsub && subgroups[l] != members) ++l;
+            if (l == nsub) {
+                if (nsub == subgroups.size()) {
+                    subgroups.push_back(members);
+                    sub_frames.push_back(eval_frame(members, G.rep));
+                } else {
+                    subgroups[nsub] = members;
+                    sub_frames[nsub] = eval_frame(members, G.rep);
+                }
+                ++nsub;
             }
             Frame fr = sub_frames[l];
             fr.orient = G.orient;
@@ -326,13 +396,12 @@ bool generate_tangents(const MeshView& mv, std::vector<float>& out) {
     }
     // 7. degenerate triangles copy the first good corner with the same welded vertex
     if (ngood < nf) {
-        std::unordered_map<int, int> first_good;  // welded vertex -> list slot
-        for (int j = 3 * ngood - 1; j >= 0; --j) first_good[tl[j]] = j;
+        std::vector<int> first_good(4 * (size_t)nf, -1);  // welded vertex (a corner idx) -> list slot
+        for (int j = 3 * ngood - 1; j >= 0; --j) first_good[(size_t)tl[j]] = j;
         for (int t = ngood; t < nf; ++t)
             for (int i = 0; i < 3; ++i) {
-                auto it = first_good.find(tl[3 * t + i]);
-                if (it == first_good.end()) continue;
-                const int src = it->second;
+                const int src = first_good[(size_t)tl[3 * t + i]];
+                if (src < 0) continue;
                 frames[3 * (size_t)tris[t].face + i] = frames[3 * (size_t)tris[src / 3].face + src % 3];
             }
     }

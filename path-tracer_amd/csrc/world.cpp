@@ -31,9 +31,13 @@
 #include <thread>
 #include <vector>
 
+#include "host_parallel.h"
 #include "pt_internal.h"
 
 namespace {
+
+using ptcore::host_threads;
+using ptcore::run_parallel;
 
 // fmin/fmax as the reference's f32::min/max: a NaN operand gives the other
 // one; of equal operands (+0 and -0) the second, as glibc's fminf/fmaxf give.
@@ -73,17 +77,6 @@ inline size_t f32_as_usize(float v) {
     return (size_t)v;
 }
 
-// Worker threads for host preparation: PT_HOST_THREADS when set (1 builds on
-// the calling thread alone), else the hardware's count capped at 16.
-unsigned host_threads() {
-    if (const char* v = std::getenv("PT_HOST_THREADS")) {
-        long t = std::strtol(v, nullptr, 10);
-        if (t >= 1) return (unsigned)std::min(t, 256L);
-    }
-    unsigned hw = std::thread::hardware_concurrency();
-    return std::max(1u, std::min(hw, 16u));
-}
-
 // Triangles below which a BVH pass is not cut into chunks, and twice which a
 // subtree is not split further before the workers take over:
 // PT_BVH_TASK_GRAIN when set, else 4096.
@@ -100,36 +93,6 @@ size_t bvh_task_grain() {
 // ends (it overflows its stack): non-finite SAH costs from NaN or
 // overflowing vertex coordinates lead there.
 struct BvhNoProgress {};
-
-// Run f(t) for t in [0, threads) on as many threads, the caller taking t = 0;
-// the first exception any of them threw is rethrown after all have joined.
-template <class F>
-void run_parallel(unsigned threads, F&& f) {
-    if (threads <= 1) {
-        f(0u);
-        return;
-    }
-    std::vector<std::exception_ptr> errors(threads);
-    auto guarded = [&](unsigned t) {
-        try {
-            f(t);
-        } catch (...) {
-            errors[t] = std::current_exception();
-        }
-    };
-    std::vector<std::thread> pool;
-    pool.reserve(threads - 1);
-    try {
-        for (unsigned t = 1; t < threads; ++t) pool.emplace_back(guarded, t);
-    } catch (...) {
-        for (std::thread& th : pool) th.join();
-        throw;
-    }
-    guarded(0u);
-    for (std::thread& th : pool) th.join();
-    for (std::exception_ptr& e : errors)
-        if (e) std::rethrow_exception(e);
-}
 
 // The BVH builder of world.rs:1430-1591: binned SAH over centroid bounds with
 // PT_BVH_BINS bins, leaf at <= 2 triangles or when the best split costs at

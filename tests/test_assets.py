@@ -323,6 +323,76 @@ def test_mikktspace_frames_on_a_uv_sphere_are_orthonormal():
         assert np.all(at[k][:, 3] == 1.0)
 
 
+def awkward_grid(n, seed):
+    """An n x n heightfield with a mirrored-uv half, a collapsed-uv band, some
+    reversed windings, a NaN normal and degenerate triangles. Its last rows use
+    duplicates of their vertices, which must weld with the originals."""
+    rng = np.random.default_rng(seed)
+    ys, xs = np.mgrid[0:n, 0:n].astype(np.float32)
+    P = np.stack([xs / n, ys / n, rng.normal(0, 0.05, (n, n)).astype(np.float32)], -1).reshape(-1, 3)
+    UV = np.stack([xs / n, ys / n], -1).reshape(-1, 2)
+    UV[xs.reshape(-1) > n / 2, 0] = 1.0 - UV[xs.reshape(-1) > n / 2, 0]
+    UV[ys.reshape(-1) < 0.15 * n] = 0.25
+    N = np.tile(np.float32([0, 0, 1]), (n * n, 1)) + rng.normal(0, 0.1, (n * n, 3)).astype(np.float32)
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    N[n + 1] = np.nan
+    x, y = np.meshgrid(np.arange(n - 1), np.arange(n - 1))
+    a = (y * n + x).reshape(-1)
+    quads = np.stack([a, a + 1, a + n + 1, a, a + n + 1, a + n], 1).reshape(-1, 3)
+    flip = rng.random(len(quads)) < 0.1
+    quads[flip] = quads[flip][:, ::-1]
+    dup_from = n * (n - 8)
+    P = np.concatenate([P, P[dup_from:]])
+    N = np.concatenate([N, N[dup_from:]])
+    UV = np.concatenate([UV, UV[dup_from:]])
+    tail = quads.min(1) >= dup_from
+    quads[tail] += n * n - dup_from
+    I = np.concatenate([quads.reshape(-1), [0, 0, 1, 5, 6, 5]]).astype(np.uint32)
+    return P.astype(np.float32), N.astype(np.float32), UV.astype(np.float32), I
+
+
+def test_mikktspace_weld_and_thread_count():
+    """The weld and the derivative pass run on several threads above 65536
+    vertices: the frames must be the same on one thread, and corners of
+    duplicated vertices must carry the frames of the originals they weld with."""
+    n = 300
+    P, N, UV, I = awkward_grid(n, 3)
+    g = GlbBuilder()
+    attrs = {"POSITION": g.accessor(P), "NORMAL": g.accessor(N), "TEXCOORD_0": g.accessor(UV)}
+    for k in ("min", "max"):  # optional for normals, and JSON has no NaN
+        g.doc["accessors"][attrs["NORMAL"]].pop(k)
+    g.doc["meshes"] = [{"primitives": [{"attributes": attrs, "indices": g.accessor(I)}]}]
+    g.doc["nodes"] = [{"mesh": 0}]
+    glb = g.build()
+    out = {}
+    saved = os.environ.get("PT_HOST_THREADS")
+    try:
+        for t in ("1", "8"):
+            os.environ["PT_HOST_THREADS"] = t
+            _, arr = load(glb)
+            out[t] = arr["attributes"].copy()
+    finally:
+        if saved is None:
+            os.environ.pop("PT_HOST_THREADS", None)
+        else:
+            os.environ["PT_HOST_THREADS"] = saved
+    assert out["1"].tobytes() == out["8"].tobytes()
+    # the same mesh without duplicates: every corner's frame is unchanged
+    dup_from = n * (n - 8)
+    I2 = I.copy()
+    I2[I2 >= n * n] -= n * n - dup_from
+    g2 = GlbBuilder()
+    attrs2 = {"POSITION": g2.accessor(P[:n * n]), "NORMAL": g2.accessor(N[:n * n]),
+              "TEXCOORD_0": g2.accessor(UV[:n * n])}
+    for k in ("min", "max"):
+        g2.doc["accessors"][attrs2["NORMAL"]].pop(k)
+    g2.doc["meshes"] = [{"primitives": [{"attributes": attrs2, "indices": g2.accessor(I2)}]}]
+    g2.doc["nodes"] = [{"mesh": 0}]
+    _, arr2 = load(g2.build())
+    for k in ("t0", "t1", "t2"):
+        assert np.array_equal(arr2["attributes"][k], out["8"][k], equal_nan=True)
+
+
 @pytest.mark.skipif(not os.path.exists(REF), reason="the reference's asset files are not present")
 def test_reference_default_assets_load():
     a = ptcore.Assets(os.path.join(REF, "glb", "box.glb"), os.path.join(REF, "skybox", "kloppenheim_06_puresky_2k.hdr"))
