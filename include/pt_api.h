@@ -337,7 +337,9 @@ pt_status pt_autotune_batch(pt_tracer* const* tracers, uint32_t count, const uin
  * in sample order, so its image is bit-identical however it is mixed. */
 typedef struct pt_server pt_server;
 typedef struct pt_server_config {
-    uint64_t hbm_budget_bytes;  /* admission budget: resident scenes + job buffers + path state */
+    uint64_t hbm_budget_bytes;  /* admission budget: resident scenes + job buffers, + the worker's
+                                   path state while another job runs (a job that fits alone
+                                   always runs) */
     uint32_t max_active_jobs;   /* job slots */
     uint32_t round_paths;       /* paths per round (0: the context's chunk size) */
 } pt_server_config;
@@ -377,9 +379,14 @@ pt_status pt_server_submit(pt_server* server, const pt_render_request* request, 
 pt_status pt_server_wait(pt_server* server, uint64_t job_id, uint32_t timeout_ms);
 pt_status pt_server_info(pt_server* server, uint64_t job_id, pt_job_info* info);
 /* Copy a done job's accumulation (width*height float4, the sum of its
- * samples) to host memory and release the job. */
+ * samples) to host memory and release the job. For a failed or cancelled
+ * job, return PT_ERR_DEVICE or PT_ERR_CANCELLED and release it (rgba is not
+ * written); PT_ERR_STATE while it is queued or running. */
 pt_status pt_server_result(pt_server* server, uint64_t job_id, float* rgba);
-/* Cancel a queued job (running jobs finish their current round first). */
+/* Cancel a job. A queued job is cancelled at once. A running job is taken
+ * out of scheduling and cancelled, with its buffer freed, once the round in
+ * flight ends; if that round completes it, it is done instead. PT_ERR_STATE
+ * for a finished job. */
 pt_status pt_server_cancel(pt_server* server, uint64_t job_id);
 pt_status pt_server_stats_get(pt_server* server, pt_server_stats* out);
 

@@ -15,6 +15,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -184,6 +185,7 @@ struct ServerJob {
     uint32_t blocks = 0;        // 8x8 blocks of the image
     uint32_t next_block = 0;    // first block not yet scheduled
     uint32_t in_flight = 0;     // blocks scheduled in the running round
+    bool cancel = false;        // a running job asked to stop after its round
     std::string error;
 };
 
@@ -221,11 +223,26 @@ struct pt_server {
         return b;
     }
 
-    // whether the first queued job fits a slot and the budget now; under mu
+    // whether the first queued job fits a slot and the budget now; under mu.
+    // The worker's path-state scratch counts only beside running jobs: a job
+    // that fits the budget alone (checked at submission) always gets to run,
+    // so the queue cannot stall behind scratch that no job is using.
     bool front_fits() const {
         if (queued.empty() || active.size() >= cfg.max_active_jobs) return false;
         const ServerJob& j = jobs.at(queued.front());
-        return bytes_in_use + job_bytes(j.req) + scene_bytes_with(j.req.scene) + arena_bytes <= cfg.hbm_budget_bytes;
+        const uint64_t scratch = active.empty() ? 0 : arena_bytes;
+        return bytes_in_use + job_bytes(j.req) + scene_bytes_with(j.req.scene) + scratch <= cfg.hbm_budget_bytes;
+    }
+
+    // a job leaves its slot: its buffer is released now, its record stays
+    // until the caller collects it; under mu
+    void finish(ServerJob& j, int32_t state, double t) {
+        if (j.accum) (void)hipFree(j.accum);
+        j.accum = nullptr;
+        bytes_in_use -= j.footprint;
+        j.footprint = 0;
+        j.state = state;
+        j.done_s = t;
     }
 
     // move queued jobs into slots while the budget allows (FIFO); under mu
@@ -234,7 +251,16 @@ struct pt_server {
             ServerJob& j = jobs.at(queued.front());
             const uint64_t need = job_bytes(j.req);
             hipError_t e = hipMalloc(&j.accum, need);
-            if (e != hipSuccess) break;  // retry when memory frees up
+            if (e != hipSuccess) {
+                j.accum = nullptr;
+                if (!active.empty()) break;  // retry when a running job frees memory
+                // nothing here will free memory: the job cannot start
+                j.error = std::string("device allocation of the job buffer failed: ") + hipGetErrorString(e);
+                finish(j, PT_JOB_FAILED, now_s());
+                queued.pop_front();
+                cv_done.notify_all();
+                continue;
+            }
             j.footprint = need;
             bytes_in_use += need;
             j.state = PT_JOB_RUNNING;
@@ -245,18 +271,39 @@ struct pt_server {
         }
     }
 
+    // drop finished and cancelled jobs from the slots; under mu
+    void retire() {
+        const double t = now_s();
+        bool any = false;
+        for (uint64_t id : active) {
+            ServerJob& j = jobs.at(id);
+            if (j.state == PT_JOB_RUNNING && j.cancel && j.in_flight == 0) {
+                finish(j, PT_JOB_CANCELLED, t);
+                any = true;
+            }
+        }
+        active.erase(std::remove_if(active.begin(), active.end(),
+                                    [&](uint64_t id) { return jobs.at(id).state != PT_JOB_RUNNING; }),
+                     active.end());
+        if (any) cv_done.notify_all();
+    }
+
     void run() {
         (void)hipSetDevice(ctx->device);  // a failure surfaces as the first round's error
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
+            retire();
             admit();
             cv_work.wait(lk, [&] {
                 if (stop) return true;
-                for (uint64_t id : active)
-                    if (jobs.at(id).next_block < jobs.at(id).blocks) return true;
+                for (uint64_t id : active) {
+                    const ServerJob& j = jobs.at(id);
+                    if (j.cancel || j.next_block < j.blocks) return true;
+                }
                 return front_fits();
             });
             if (stop) return;
+            retire();
             admit();
             // one round: blocks from the active jobs in admission order
             const uint32_t round_paths = cfg.round_paths ? cfg.round_paths : ctx->wave_cfg.chunk_paths;
@@ -265,7 +312,7 @@ struct pt_server {
             uint64_t paths = 0;
             for (uint64_t id : active) {
                 ServerJob& j = jobs.at(id);
-                if (j.next_block >= j.blocks) continue;
+                if (j.cancel || j.next_block >= j.blocks) continue;
                 const uint64_t per_block = 64ull * j.req.samples;
                 uint64_t room = paths >= round_paths ? 0 : (round_paths - paths) / per_block;
                 if (room == 0 && !round.empty()) break;
@@ -307,21 +354,15 @@ struct pt_server {
                 ServerJob& j = jobs.at(id);
                 j.in_flight = 0;
                 if (st != PT_OK || e != hipSuccess) {
-                    j.state = PT_JOB_FAILED;
                     j.error = st != PT_OK ? "render failed" : hipGetErrorString(e);
-                    j.done_s = t1;
+                    finish(j, PT_JOB_FAILED, t1);
                 } else if (j.next_block >= j.blocks) {
                     j.state = PT_JOB_DONE;
                     j.done_s = t1;
                     stats.jobs_done += 1;
                 }
             }
-            active.erase(std::remove_if(active.begin(), active.end(),
-                                        [&](uint64_t id) {
-                                            int s = jobs.at(id).state;
-                                            return s == PT_JOB_DONE || s == PT_JOB_FAILED;
-                                        }),
-                         active.end());
+            retire();
             cv_done.notify_all();
         }
     }
@@ -427,7 +468,7 @@ pt_status pt_server_info(pt_server* s, uint64_t id, pt_job_info* info) {
 }
 
 pt_status pt_server_result(pt_server* s, uint64_t id, float* rgba) {
-    if (!s || !rgba) return set_error(PT_ERR_INVALID, "null argument");
+    if (!s) return set_error(PT_ERR_INVALID, "null argument");
     float4* buf = nullptr;
     size_t bytes = 0;
     {
@@ -435,7 +476,16 @@ pt_status pt_server_result(pt_server* s, uint64_t id, float* rgba) {
         auto it = s->jobs.find(id);
         if (it == s->jobs.end()) return set_error(PT_ERR_INVALID, "unknown job %llu", (unsigned long long)id);
         ServerJob& j = it->second;
-        if (j.state != PT_JOB_DONE) return set_error(PT_ERR_STATE, "job %llu is not done", (unsigned long long)id);
+        if (j.state == PT_JOB_FAILED || j.state == PT_JOB_CANCELLED) {
+            // the record is released with the outcome
+            const bool failed = j.state == PT_JOB_FAILED;
+            const std::string why = j.error;
+            s->jobs.erase(it);
+            if (failed) return set_error(PT_ERR_DEVICE, "job %llu failed: %s", (unsigned long long)id, why.c_str());
+            return set_error(PT_ERR_CANCELLED, "job %llu was cancelled", (unsigned long long)id);
+        }
+        if (j.state != PT_JOB_DONE) return set_error(PT_ERR_STATE, "job %llu is not finished", (unsigned long long)id);
+        if (!rgba) return set_error(PT_ERR_INVALID, "null argument");
         buf = j.accum;
         bytes = j.footprint;
         j.accum = nullptr;
@@ -454,10 +504,19 @@ pt_status pt_server_cancel(pt_server* s, uint64_t id) {
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->jobs.find(id);
     if (it == s->jobs.end()) return set_error(PT_ERR_INVALID, "unknown job %llu", (unsigned long long)id);
-    if (it->second.state != PT_JOB_QUEUED) return set_error(PT_ERR_STATE, "only queued jobs can be cancelled");
-    it->second.state = PT_JOB_CANCELLED;
-    s->queued.erase(std::remove(s->queued.begin(), s->queued.end(), id), s->queued.end());
-    s->cv_done.notify_all();
+    ServerJob& j = it->second;
+    if (j.state == PT_JOB_QUEUED) {
+        j.state = PT_JOB_CANCELLED;
+        j.done_s = now_s();
+        s->queued.erase(std::remove(s->queued.begin(), s->queued.end(), id), s->queued.end());
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();  // the next queued job may now be first in line
+        return PT_OK;
+    }
+    if (j.state != PT_JOB_RUNNING) return set_error(PT_ERR_STATE, "job %llu has finished", (unsigned long long)id);
+    // the worker stops scheduling it and releases it after its round in flight
+    j.cancel = true;
+    s->cv_work.notify_all();
     return PT_OK;
 }
 

@@ -7,6 +7,7 @@ the path computation is deterministic and each pixel's samples are always
 accumulated in sample order.
 """
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -186,10 +187,61 @@ def test_server_rejects_over_budget_and_cancels(ctx, scenes):
         srv.wait(queued[2])
         assert srv.info(queued[1])["state"] == ptcore.JOB_CANCELLED
         with pytest.raises(ptcore.PtError) as e:
-            srv.result(queued[1], 64, 64)
+            srv.cancel(queued[1])
         assert e.value.code == ptcore.PT_ERR_STATE
+        # collecting a cancelled job reports it and releases the record
+        with pytest.raises(ptcore.PtError) as e:
+            srv.result(queued[1], 64, 64)
+        assert e.value.code == ptcore.PT_ERR_CANCELLED
+        with pytest.raises(ptcore.PtError) as e:
+            srv.info(queued[1])
+        assert e.value.code == ptcore.PT_ERR_INVALID
     finally:
         srv.close()
+
+
+def test_server_idle_scratch_does_not_block_admission(ctx, scenes):
+    """The budget fits one job with its scene but not the path state the
+    worker keeps after a round: the next job must still run once the server
+    is idle."""
+    _, sc = scenes
+    srv = ptcore.Server(ctx, hbm_budget_bytes=sc[0].footprint() + 64 * 64 * 16 + (1 << 16), max_active_jobs=1,
+                        round_paths=1 << 12)
+    try:
+        a = srv.submit(sc[0], 64, 64, 2, 2)
+        srv.wait(a, 30_000)
+        first = srv.result(a, 64, 64)
+        b = srv.submit(sc[0], 64, 64, 2, 2)
+        srv.wait(b, 30_000)
+        second = srv.result(b, 64, 64)
+    finally:
+        srv.close()
+    assert np.array_equal(first, second)
+
+
+def test_server_cancels_a_running_job(ctx, scenes):
+    """A running job stops after its round in flight and frees its buffer;
+    jobs behind it then run, and their images are unaffected."""
+    _, sc = scenes
+    srv = ptcore.Server(ctx, hbm_budget_bytes=1 << 34, max_active_jobs=1, round_paths=1 << 12)
+    try:
+        big = srv.submit(sc[2], 512, 512, 64, 6)  # 16.8M paths in 4096-path rounds
+        small = srv.submit(sc[0], 64, 64, 2, 2)
+        deadline = time.time() + 30
+        while srv.info(big)["state"] != ptcore.JOB_RUNNING and time.time() < deadline:
+            time.sleep(0.001)
+        assert srv.info(big)["state"] == ptcore.JOB_RUNNING
+        srv.cancel(big)
+        with pytest.raises(ptcore.PtError) as e:
+            srv.wait(big)
+        assert e.value.code == ptcore.PT_ERR_CANCELLED
+        srv.wait(small)
+        got = srv.result(small, 64, 64)
+        stats = srv.stats()
+    finally:
+        srv.close()
+    assert stats["bytes_in_use"] == 0, "the cancelled job's buffer must be released"
+    assert np.array_equal(got, render_alone(ctx, sc, [(0, 64, 64, 2, 2)])[0])
 
 
 def test_server_wait_times_out(ctx, scenes):
