@@ -11,7 +11,8 @@ Prints one JSON line: path samples/s (value), rays/s, p50/p99 job latency,
 the roofline of the traversal (extend) kernel measured live with HIP events
 on its stream, and the CPU oracle's throughput on the host cores.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload field|cornell|large]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload field|cornell|large|batch|serve|interactive]
                     [--schedule wavefront|mega] [--no-cpu-baseline]
 """
 import argparse
@@ -253,12 +254,89 @@ def bench_serve(args):
             "sweep": results}
 
 
+def wave_knobs(args):
+    """The wavefront configuration fields set on the command line."""
+    return {k: v for k, v in (("traversal", args.traversal), ("sort_key", args.sort), ("lds_stack_depth", args.lds),
+                              ("streams", args.streams), ("chunk_paths", args.chunk))
+            if v is not None}
+
+
+def bench_interactive(args):
+    """Progressive frames as the reference's interactive mode renders them
+    (interactive.rs:554, 611): each frame traces one more sample of every
+    pixel (PathTracer::trace with n = 1) and tone-maps the running image for
+    display. The RGBA8 frame is copied to host memory here; a window's swap
+    chain would take it instead. One step is one frame; both schedules run."""
+    import numpy as np
+    import ptcore
+
+    W, H, BOUNCES = 1920, 1080, 6
+    frames = args.steps if args.steps is not None else 200
+    warm = args.warmup if args.warmup is not None else 20
+    world = ptcore.World.generate(ptcore.SCENE_FIELD, 2, W, H)
+    ctx = ptcore.Context(0)
+    scene = ptcore.Scene(ctx, world.desc)
+    knobs = wave_knobs(args)
+    if knobs:
+        ptcore.set_wave_config(ctx, **{**ptcore.get_wave_config(ctx), **knobs})
+    frame = np.empty((H, W, 4), np.uint8)
+    per_schedule = {}
+    for name, schedule in (("wavefront", ptcore.SCHEDULE_WAVEFRONT), ("mega", ptcore.SCHEDULE_MEGAKERNEL)):
+        tracer = ptcore.Tracer(ctx, scene, W, H, schedule)
+        tracer.set_max_bounces(BOUNCES)
+
+        def one_frame():
+            tracer.trace(1)
+            tracer.display_rgba8(frame)
+
+        for _ in range(warm):
+            one_frame()
+        tracer.set_sample_count(0)
+        ctx.synchronize()
+        lat = []
+        t_start = time.perf_counter()
+        for _ in range(frames):
+            s0 = time.perf_counter()
+            one_frame()  # returns after the frame is in host memory
+            lat.append((time.perf_counter() - s0) * 1e3)
+        total_s = time.perf_counter() - t_start
+        # the same frames without the display copy: trace, then wait for the device
+        tracer.set_sample_count(0)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            tracer.trace(1)
+            ctx.synchronize()
+        trace_only_ms = (time.perf_counter() - t0) * 1e3 / frames
+        per_schedule[name] = {"frames_per_s": frames / total_s, "ms_per_frame": total_s * 1e3 / frames,
+                              "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+                              "path_samples_per_s": W * H * frames / total_s,
+                              "trace_only_ms_per_frame": trace_only_ms}
+        print(f"[bench] interactive {name}: {per_schedule[name]}", file=sys.stderr, flush=True)
+        tracer.close()
+    best = max(per_schedule, key=lambda k: per_schedule[k]["frames_per_s"])
+    out = {
+        "metric": "interactive frames/s @1080p depth 6 (1 spp per frame + display)",
+        "value": per_schedule[best]["frames_per_s"], "unit": "frames/s", "n_gpus": 1, "steps": frames, "warmup": warm,
+        "ms_per_step": per_schedule[best]["ms_per_frame"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": "interactive: field-262k, 1920x1080, one sample per frame, depth 6, tone-mapped "
+                               "RGBA8 frame to host each frame", "schedule": best,
+                   "wave_config": ptcore.get_wave_config(ctx)},
+        "schedules": per_schedule,
+    }
+    scene.close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="field", choices=sorted(WORKLOADS) + ["batch", "serve", "copy"])
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 5; interactive: 200 frames)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 2; interactive: 20 frames)")
+    ap.add_argument("--workload", default="field",
+                    choices=sorted(WORKLOADS) + ["batch", "serve", "copy", "interactive"])
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--rates", type=float, nargs="+", default=[5, 10, 20, 50, 100, 200])
@@ -282,6 +360,11 @@ def main():
 
     import ptcore
 
+    if args.workload == "interactive":
+        print(json.dumps(bench_interactive(args)), flush=True)
+        return
+    args.steps = 5 if args.steps is None else args.steps
+    args.warmup = 2 if args.warmup is None else args.warmup
     if args.workload in ("batch", "serve"):
         out = bench_batch(args) if args.workload == "batch" else bench_serve(args)
         print(json.dumps(out), flush=True)
@@ -308,9 +391,7 @@ def main():
     ctx.synchronize()
     upload_s = time.perf_counter() - t0
     schedule = ptcore.SCHEDULE_WAVEFRONT if args.schedule == "wavefront" else ptcore.SCHEDULE_MEGAKERNEL
-    knobs = {k: v for k, v in (("traversal", args.traversal), ("sort_key", args.sort), ("lds_stack_depth", args.lds),
-                               ("streams", args.streams), ("chunk_paths", args.chunk))
-             if v is not None}
+    knobs = wave_knobs(args)
     if knobs:
         ptcore.set_wave_config(ctx, **{**ptcore.get_wave_config(ctx), **knobs})
     tracer = ptcore.Tracer(ctx, scene, W, H, schedule)

@@ -48,6 +48,7 @@ pt_status batch_jobs(pt_tracer* const* tracers, uint32_t count, const uint32_t* 
         pt_tracer* t = tracers[i];
         if (!t) return set_error(PT_ERR_INVALID, "null tracer %u", i);
         if (t->ctx != c) return set_error(PT_ERR_INVALID, "tracers of a batch must share a context");
+        if (pt_status ts = tracer_require_tile(t)) return ts;
         for (uint32_t k = 0; k < i; ++k)
             if (tracers[k] == t) return set_error(PT_ERR_INVALID, "tracer %u appears twice in the batch", i);
         tracer_sync_version(t);

@@ -24,6 +24,12 @@ void tracer_sync_version(pt_tracer* t) {
     }
 }
 
+pt_status tracer_require_tile(const pt_tracer* t) {
+    if (!t->accum || !t->storage)
+        return set_error(PT_ERR_STATE, "tracer has no tile buffers (a failed set_tile released them; set a tile again)");
+    return PT_OK;
+}
+
 HostJob tracer_job(pt_tracer* t, uint32_t samples) {
     HostJob hj;
     hj.scene = t->scene;
@@ -47,18 +53,45 @@ HostJob tracer_job(pt_tracer* t, uint32_t samples) {
 
 namespace {
 
+// (Re)allocate the tile buffers. On failure the tracer is left with none and
+// a zero tile, so no later launch can address a buffer of another size.
 pt_status alloc_tile(pt_tracer* t, uint32_t w, uint32_t h) {
     (void)hipFree(t->accum);
     (void)hipFree(t->storage);
     t->accum = t->storage = nullptr;
+    t->tile_w = t->tile_h = 0;
     size_t n = (size_t)w * h * sizeof(float4);
     hipError_t e = hipMalloc(&t->accum, n);
     if (e == hipSuccess) e = hipMalloc(&t->storage, n);
     if (e == hipSuccess) e = hipMemsetAsync(t->accum, 0, n, t->ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(t->storage, 0, n, t->ctx->stream);
-    if (e != hipSuccess) return hip_error(e, "tile buffers");
+    if (e != hipSuccess) {
+        (void)hipFree(t->accum);
+        (void)hipFree(t->storage);
+        t->accum = t->storage = nullptr;
+        return hip_error(e, "tile buffers");
+    }
     t->tile_w = w;
     t->tile_h = h;
+    return PT_OK;
+}
+
+// device staging of at least `bytes`, kept across calls
+pt_status tracer_scratch(pt_tracer* t, size_t bytes, void** out) {
+    if (bytes > t->scratch_bytes) {
+        hipError_t se = hipStreamSynchronize(t->ctx->stream);
+        if (se != hipSuccess) return hip_error(se, "tracer staging");
+        (void)hipFree(t->scratch);
+        t->scratch = nullptr;
+        t->scratch_bytes = 0;
+        hipError_t e = hipMalloc(&t->scratch, bytes);
+        if (e != hipSuccess) {
+            t->scratch = nullptr;
+            return hip_error(e, "tracer staging");
+        }
+        t->scratch_bytes = bytes;
+    }
+    *out = t->scratch;
     return PT_OK;
 }
 
@@ -89,6 +122,7 @@ ptd::TileArgs tile_args(const pt_tracer* t, uint32_t first_sample, uint32_t samp
 
 pt_status trace_samples(pt_tracer* t, uint32_t samples) {
     Context* c = t->ctx;
+    if (pt_status ts = tracer_require_tile(t)) return ts;
     if (pt_status dst = use_device(c->device)) return dst;
     sync_version(t);
     const uint32_t first = t->params.sample_count + 1u;
@@ -173,6 +207,7 @@ pt_status pt_tracer_destroy(pt_tracer* t) {
     delete t->wave;
     (void)hipFree(t->accum);
     (void)hipFree(t->storage);
+    (void)hipFree(t->scratch);
     delete t;
     return pending == hipSuccess ? PT_OK : hip_error(pending, "work pending at tracer destruction");
 }
@@ -244,6 +279,7 @@ pt_status pt_tracer_trace_all(pt_tracer* t, uint32_t samples) {
 
 pt_status pt_tracer_read_accumulation(pt_tracer* t, float* rgba) {
     if (!t || !rgba) return set_error(PT_ERR_INVALID, "null argument");
+    if (pt_status ts = tracer_require_tile(t)) return ts;
     if (pt_status dst = use_device(t->ctx->device)) return dst;
     hipError_t e = hipMemcpyAsync(rgba, t->accum, (size_t)t->tile_w * t->tile_h * sizeof(float4), hipMemcpyDeviceToHost,
                                   t->ctx->stream);
@@ -253,6 +289,7 @@ pt_status pt_tracer_read_accumulation(pt_tracer* t, float* rgba) {
 
 pt_status pt_tracer_write_accumulation(pt_tracer* t, const float* rgba) {
     if (!t || !rgba) return set_error(PT_ERR_INVALID, "null argument");
+    if (pt_status ts = tracer_require_tile(t)) return ts;
     if (pt_status dst = use_device(t->ctx->device)) return dst;
     hipError_t e = hipMemcpyAsync(t->accum, rgba, (size_t)t->tile_w * t->tile_h * sizeof(float4), hipMemcpyHostToDevice,
                                   t->ctx->stream);
@@ -262,6 +299,7 @@ pt_status pt_tracer_write_accumulation(pt_tracer* t, const float* rgba) {
 
 pt_status pt_tracer_read_storage(pt_tracer* t, float* rgba) {
     if (!t || !rgba) return set_error(PT_ERR_INVALID, "null argument");
+    if (pt_status ts = tracer_require_tile(t)) return ts;
     if (pt_status dst = use_device(t->ctx->device)) return dst;
     hipError_t e = hipMemcpyAsync(rgba, t->storage, (size_t)t->tile_w * t->tile_h * sizeof(float4), hipMemcpyDeviceToHost,
                                   t->ctx->stream);
@@ -271,13 +309,14 @@ pt_status pt_tracer_read_storage(pt_tracer* t, float* rgba) {
 
 pt_status pt_tracer_display_rgba8(pt_tracer* t, uint8_t* rgba) {
     if (!t || !rgba) return set_error(PT_ERR_INVALID, "null argument");
+    if (pt_status ts = tracer_require_tile(t)) return ts;
     if (pt_status dst = use_device(t->ctx->device)) return dst;
     const uint32_t n = t->tile_w * t->tile_h;
-    DeviceBuffer buf;
-    pt_status st = buf.alloc((size_t)n * 4);
+    void* buf = nullptr;
+    pt_status st = tracer_scratch(t, (size_t)n * 4, &buf);
     if (st) return st;
-    hipError_t e = launch_display(t->storage, n, (uchar4*)buf.ptr, t->ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(rgba, buf.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, t->ctx->stream);
+    hipError_t e = launch_display(t->storage, n, (uchar4*)buf, t->ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(rgba, buf, (size_t)n * 4, hipMemcpyDeviceToHost, t->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(t->ctx->stream);
     return e == hipSuccess ? PT_OK : hip_error(e, "display");
 }
@@ -285,6 +324,7 @@ pt_status pt_tracer_display_rgba8(pt_tracer* t, uint8_t* rgba) {
 pt_status pt_tracer_primary_hits(pt_tracer* t, uint32_t sample_count, int32_t* tri, float* tt) {
     if (!t || !tri || !tt) return set_error(PT_ERR_INVALID, "null argument");
     if (sample_count == 0) return set_error(PT_ERR_INVALID, "sample_count is 1-based");
+    if (pt_status ts = tracer_require_tile(t)) return ts;
     Context* c = t->ctx;
     if (pt_status dst = use_device(c->device)) return dst;
     const uint32_t n = t->tile_w * t->tile_h;
@@ -292,9 +332,9 @@ pt_status pt_tracer_primary_hits(pt_tracer* t, uint32_t sample_count, int32_t* t
     const uint32_t grid = std::min<uint32_t>(blocks, c->cu_count * 8u);
     pt_status st = c->ensure_overflow(t->scene->bvh_depth + 1u, grid * 256u);
     if (st) return st;
-    DeviceBuffer buf;
-    if ((st = buf.alloc((size_t)n * 8))) return st;
-    int32_t* dtri = (int32_t*)buf.ptr;
+    void* buf = nullptr;
+    if ((st = tracer_scratch(t, (size_t)n * 8, &buf))) return st;
+    int32_t* dtri = (int32_t*)buf;
     float* dt = (float*)(dtri + n);
     ptd::TileArgs a = tile_args(t, sample_count, 1);
     hipError_t e = launch_primary_hits(t->scene->view, a, dtri, dt, c->ovf, grid, c->stream);

@@ -16,6 +16,8 @@ struct pt_tracer {
     ptcore::EventLog log;                    // kernel events since set_profiling
     unsigned long long counters[6] = {0};    // summed device counters since set_profiling
     ptcore::Wavefront* wave = nullptr;
+    void* scratch = nullptr;                 // display / primary-hit staging, grown on demand
+    size_t scratch_bytes = 0;
 };
 
 namespace ptcore {
@@ -23,4 +25,6 @@ namespace ptcore {
 void tracer_sync_version(pt_tracer* t);
 // The wavefront job of the tracer's next `samples` samples.
 HostJob tracer_job(pt_tracer* t, uint32_t samples);
+// PT_ERR_STATE when the tracer has no tile buffers (a failed resize freed them).
+pt_status tracer_require_tile(const pt_tracer* t);
 }  // namespace ptcore

@@ -800,8 +800,12 @@ class Tracer:
         _check(lib().pt_tracer_read_storage(self.handle, _ptr(out)))
         return out
 
-    def display_rgba8(self):
-        out = np.empty((self.tile_h, self.tile_w, 4), np.uint8)
+    def display_rgba8(self, out=None):
+        """Tone-mapped RGBA8 tile; `out` (tile_h x tile_w x 4 uint8, contiguous) is reused when given."""
+        if out is None:
+            out = np.empty((self.tile_h, self.tile_w, 4), np.uint8)
+        if out.dtype != np.uint8 or out.shape != (self.tile_h, self.tile_w, 4) or not out.flags.c_contiguous:
+            raise ValueError("display buffer must be a contiguous tile_h x tile_w x 4 uint8 array")
         _check(lib().pt_tracer_display_rgba8(self.handle, _ptr(out)))
         return out
 
