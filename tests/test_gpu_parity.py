@@ -181,3 +181,44 @@ def test_alpha_radiance_cornell(ctx, alpha_cornell, schedule):
     ref, _, _ = oracle.render_f32(w.desc, 256, 256, bounces, spp)
     assert_parity(gpu, ref, f"alpha cornell schedule={schedule}")
     t.close()
+
+
+@pytest.mark.parametrize("schedule", [ptcore.SCHEDULE_MEGAKERNEL, ptcore.SCHEDULE_WAVEFRONT])
+def test_tile_past_the_image_edge_keeps_outside_pixels(ctx, field, field_scene, schedule):
+    """A 37x23 tile (not a multiple of the 8x8 blocks) moved from inside the
+    image to its corner, where 20x10 of its pixels remain in the image. Only
+    those are traced (path_tracer.rs:334-338, trace.slang:731-733); the others
+    keep what the previous tile position accumulated, as the reference
+    leaves them. Compared with the oracle continuing from the same buffer."""
+    iw, ih, tw, th, bounces = 1920, 1080, 37, 23, 6
+    t = ptcore.Tracer(ctx, field_scene, tw, th, schedule)
+    t.set_max_bounces(bounces)
+    t.set_tile((iw, ih), (700, 500), (tw, th))
+    t.trace(2)
+    first = t.accumulation()
+    ref_first, _, _ = oracle.render_f32(field.desc, iw, ih, bounces, 2, tile=(tw, th), origin=(700, 500))
+    assert_parity(first, ref_first, f"inner tile schedule={schedule}")
+    t.set_tile((iw, ih), (iw - 20, ih - 10), (tw, th))
+    t.trace(3)
+    second = t.accumulation()
+    t.close()
+    ref, _, _ = oracle.render_f32(field.desc, iw, ih, bounces, 3, tile=(tw, th), origin=(iw - 20, ih - 10),
+                                  accum=ref_first.copy())
+    assert_parity(second[:10, :20], ref[:10, :20], f"edge tile schedule={schedule}")
+    assert np.array_equal(second, ref), "pixels outside the image must keep their previous values"
+
+
+@pytest.mark.parametrize("bounces", [0, 1, 2])
+def test_shallow_depths_match_oracle(ctx, cornell, cornell_scene, bounces):
+    """max_bounces 0 (camera-visible emission and direct light only), 1 and 2:
+    the terminal-bounce rules (no BSDF sample at the last surface, NEE without
+    MIS weight there) under both schedules."""
+    w = h = 128
+    ref, _, _ = oracle.render_f32(cornell.desc, 256, 256, bounces, 4, tile=(w, h), origin=(64, 64))
+    for schedule in (ptcore.SCHEDULE_MEGAKERNEL, ptcore.SCHEDULE_WAVEFRONT):
+        t = ptcore.Tracer(ctx, cornell_scene, w, h, schedule)
+        t.set_tile((256, 256), (64, 64), (w, h))
+        t.set_max_bounces(bounces)
+        t.trace(4)
+        assert_parity(t.accumulation(), ref, f"depth {bounces} schedule={schedule}")
+        t.close()
