@@ -483,8 +483,16 @@ __global__ __launch_bounds__(WAVE_BLOCK, PT_SHADE_WAVES_PER_EU) void k_shade(Dev
     __shared__ uint32_t win_hits[WIN * NW], win_misses[WIN * NW];
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t count = *count_ptr;
-    const uint32_t stride = gridDim.x * WAVE_BLOCK * WIN;
-    for (uint32_t wbase = blockIdx.x * WAVE_BLOCK * WIN; wbase < count; wbase += stride) {
+    // each block takes a contiguous run of windows, so what it appends comes
+    // from one stretch of the queue: with blocks finishing roughly in launch
+    // order, the next queues keep the order of this one, and neighbouring
+    // waves of the next traversal trace neighbouring pixels
+    const uint32_t span = WAVE_BLOCK * WIN;
+    const uint32_t windows = (count + span - 1) / span;
+    const uint32_t per_block = (windows + gridDim.x - 1) / gridDim.x;
+    const uint32_t w_begin = blockIdx.x * per_block, w_end = min(w_begin + per_block, windows);
+    for (uint32_t wi = w_begin; wi < w_end; ++wi) {
+        const uint32_t wbase = wi * span;
         uint32_t pk[WIN];
         bool hk[WIN], mk[WIN];
 #pragma unroll
