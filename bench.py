@@ -416,7 +416,9 @@ def main():
     ctx.synchronize()
     print("[bench] warmup done", file=sys.stderr, flush=True)
 
-    tracer.set_profiling(1)  # live HIP events around every kernel, no extra syncs
+    # live HIP events around every traversal launch (the roofline kernel), no
+    # extra synchronisation; other kernels carry no markers in the timed steps
+    tracer.set_profiling(3)
     lat = []
     ctx.synchronize()
     t_start = time.perf_counter()
@@ -427,7 +429,27 @@ def main():
         lat.append((time.perf_counter() - s0) * 1e3)
     total_s = time.perf_counter() - t_start
     ev = tracer.stats()
+    # kernel-class breakdown (reported, not timed): a few more steps with every kernel's events
+    breakdown_steps = min(args.steps, 3)
+    tracer.set_profiling(1)
+    for _ in range(breakdown_steps):
+        step()
+    ctx.synchronize()
+    ev_all = tracer.stats()
     tracer.set_profiling(0)
+    # the same steps on one lane: with two lanes every traversal launch shares
+    # the GPU with the other lane's kernels, so its own duration reads longer
+    ev_one = None
+    if schedule == ptcore.SCHEDULE_WAVEFRONT:
+        cfg0 = ptcore.get_wave_config(ctx)
+        ptcore.set_wave_config(ctx, **{**cfg0, "streams": 1})
+        tracer.set_profiling(3)
+        for _ in range(breakdown_steps):
+            step()
+        ctx.synchronize()
+        ev_one = tracer.stats()
+        tracer.set_profiling(0)
+        ptcore.set_wave_config(ctx, **cfg0)
 
     samples_per_step = W * H * SPP
     rays_per_step = counts["closest_rays"] + counts["shadow_rays"]
@@ -468,8 +490,20 @@ def main():
                               "kernel": "k_trace (closest-hit rays of bounce b + shadow rays of bounce b-1)",
                               "avg_launch_ms": avg_launch_s * 1e3, "launches_per_step": launches_per_step,
                               "algorithmic_bytes_per_launch": bytes_per_launch}
-        result["kernel_ms_per_step"] = {"extend": ev["extend_ms"] / args.steps, "shade": ev["shade_ms"] / args.steps,
-                                        "shadow": ev["shadow_ms"] / args.steps, "total": ev["total_ms"] / args.steps}
+        one_launches = max(ev_one["extend_launches"], 1)
+        one_bytes = bytes_per_step / (one_launches / breakdown_steps)
+        one_s = ev_one["extend_ms"] / 1e3 / one_launches
+        result["roofline"]["one_lane"] = {
+            "achieved": one_bytes / one_s / 1e9, "frac": one_bytes / one_s / 1e9 / HBM_PEAK_GBS,
+            "avg_launch_ms": one_s * 1e3, "launches_per_step": one_launches / breakdown_steps,
+            "note": "the same job on one lane in separate steps; the figures above come from the timed steps, "
+                    "whose launches overlap the other lane's kernels when the render runs on two lanes"}
+        result["kernel_ms_per_step"] = {"extend": ev["extend_ms"] / args.steps,
+                                        "shade": ev_all["shade_ms"] / breakdown_steps,
+                                        "shadow": ev_all["shadow_ms"] / breakdown_steps,
+                                        "total": ev_all["total_ms"] / breakdown_steps,
+                                        "note": "extend from the timed steps; the rest from separate steps with "
+                                                "every kernel timed"}
     else:
         result["kernel_ms_per_step"] = {"megakernel": ev["total_ms"] / args.steps}
         bytes_per_step = PAIR_BYTES * counts["pair_tests"] + TRI_BYTES * counts["tri_tests"]

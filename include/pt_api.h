@@ -281,7 +281,9 @@ typedef struct pt_trace_stats {
     float total_ms;              /* whole trace calls */
 } pt_trace_stats;
 /* level 0 off, 1 per-kernel event timing, 2 timing + device counters
- * (counting kernels; synchronises once per chunk). Resets the statistics. */
+ * (counting kernels; synchronises once per chunk), 3 event timing of the
+ * traversal launches only (extend_ms; the megakernel's total_ms), which puts
+ * the fewest markers on the stream. Resets the statistics. */
 pt_status pt_tracer_set_profiling(pt_tracer* tracer, int level);
 /* Synchronises and returns the statistics accumulated since set_profiling. */
 pt_status pt_tracer_stats(pt_tracer* tracer, pt_trace_stats* out);

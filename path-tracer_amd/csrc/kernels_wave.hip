@@ -719,7 +719,7 @@ struct EventScope {
     int kind;
     hipStream_t st;
     hipEvent_t a = nullptr, b = nullptr;
-    EventScope(EventLog* l, int k, hipStream_t s) : log(l), kind(k), st(s) {
+    EventScope(EventLog* l, int k, hipStream_t s) : log(l && l->records(k) ? l : nullptr), kind(k), st(s) {
         if (log) {
             log->take(a, b);
             if (a && hipEventRecord(a, st) != hipSuccess) a = nullptr;
@@ -972,12 +972,16 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
         total_paths += (uint64_t)(be - h.block_begin) * 64u * h.job.samples;
     }
     uint32_t chunk_cap = std::max(cfg.chunk_paths, max_unit_paths);
-    // lanes: 2 on request (then at least two chunks, so they overlap), or
-    // automatically when the job spans three chunks or more: a chunk's last,
-    // small bounces then run beside the next chunk's first, large ones
-    const uint32_t lanes = cfg.streams == 2 || (cfg.streams == 0 && total_paths > 2ull * chunk_cap) ? 2u : 1u;
-    if (cfg.streams == 2 && total_paths <= chunk_cap)
-        chunk_cap = std::max<uint32_t>(max_unit_paths, (uint32_t)((total_paths + 1) / 2));
+    // lanes: a chunk's last, small bounces run beside another chunk's first,
+    // large ones, and shading beside traversal. Two lanes on request, or
+    // automatically for a render of three chunks or more, or of at least
+    // LANE_SPLIT_PATHS paths in one or two chunks, which is then cut into two
+    // equal chunks (a 1080p 8-spp job: 2.14 -> 2.20 G path samples/s).
+    constexpr uint64_t LANE_SPLIT_PATHS = 1u << 21;
+    const bool split = total_paths <= 2ull * chunk_cap &&
+                       (cfg.streams == 2 || (cfg.streams == 0 && total_paths >= LANE_SPLIT_PATHS));
+    const uint32_t lanes = cfg.streams == 2 || (cfg.streams == 0 && (split || total_paths > 2ull * chunk_cap)) ? 2u : 1u;
+    if (split) chunk_cap = std::max<uint32_t>(max_unit_paths, (uint32_t)((total_paths + 1) / 2));
     pt_status st = reserve(chunk_cap);
     if (st) return st;
     const uint32_t grid = ctx->cu_count * 8u;

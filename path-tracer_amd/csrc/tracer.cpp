@@ -138,7 +138,7 @@ pt_status trace_samples(pt_tracer* t, uint32_t samples) {
             if (me != hipSuccess) return hip_error(me, "traversal counters");
         }
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (t->profiling) {
+        if (t->profiling) {  // the megakernel's one launch, at every level
             t->log.take(e0, e1);
             if (e0 && hipEventRecord(e0, c->stream) != hipSuccess) e0 = nullptr;  // timing is best effort
         }
@@ -346,11 +346,14 @@ pt_status pt_tracer_primary_hits(pt_tracer* t, uint32_t sample_count, int32_t* t
 
 pt_status pt_tracer_set_profiling(pt_tracer* t, int level) {
     if (!t) return set_error(PT_ERR_INVALID, "null tracer");
-    if (level < 0 || level > 2) return set_error(PT_ERR_INVALID, "profiling level %d not in 0..2", level);
+    if (level < 0 || level > 3) return set_error(PT_ERR_INVALID, "profiling level %d not in 0..3", level);
     if (pt_status dst = use_device(t->ctx->device)) return dst;
     hipError_t se = hipStreamSynchronize(t->ctx->stream);
     if (se != hipSuccess) return hip_error(se, "profiling switch");
-    t->profiling = level;
+    // level 3 times only the traversal launches (the megakernel's one kernel):
+    // the fewest markers on the stream
+    t->profiling = level == 3 ? 1 : level;
+    t->log.kinds = level == 3 ? (1u << EV_EXTEND) : ~0u;
     t->log.reset();
     std::memset(t->counters, 0, sizeof(t->counters));
     return PT_OK;

@@ -75,6 +75,8 @@ struct EventLog {
     };
     std::vector<Pair> pairs;
     std::vector<hipEvent_t> pool;
+    unsigned kinds = ~0u;  // bit k: record kernels of class k (each pair is a marker on the stream)
+    bool records(int kind) const { return (kinds >> kind) & 1u; }
     void take(hipEvent_t& a, hipEvent_t& b);
     void reset();  // recycle recorded pairs
     void sum(float ms[EV_KINDS], uint32_t launches[EV_KINDS]);

@@ -222,3 +222,31 @@ def test_shallow_depths_match_oracle(ctx, cornell, cornell_scene, bounces):
         t.trace(4)
         assert_parity(t.accumulation(), ref, f"depth {bounces} schedule={schedule}")
         t.close()
+
+
+def test_profiling_levels(ctx, cornell, cornell_scene):
+    """Level 1 times every kernel class, level 3 only the traversal launches,
+    level 2 adds the device counters; the image is the same at every level."""
+    images = []
+    for level in (0, 1, 2, 3):
+        t = ptcore.Tracer(ctx, cornell_scene, 64, 64)
+        t.set_max_bounces(4)
+        t.set_profiling(level)
+        t.trace(2)
+        st = t.stats()
+        images.append(t.accumulation())
+        if level in (1, 2):
+            assert st["extend_ms"] > 0 and st["shade_ms"] > 0 and st["total_ms"] > 0
+        if level == 3:
+            assert st["extend_ms"] > 0 and st["extend_launches"] == 6
+            assert st["shade_ms"] == 0 and st["total_ms"] == 0
+        assert (st["pair_tests"] > 0) == (level == 2)
+        t.close()
+    for img in images[1:]:
+        assert np.array_equal(img, images[0])
+    with pytest.raises(ptcore.PtError):
+        t = ptcore.Tracer(ctx, cornell_scene, 8, 8)
+        try:
+            t.set_profiling(4)
+        finally:
+            t.close()
