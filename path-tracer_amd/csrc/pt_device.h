@@ -395,16 +395,31 @@ __device__ __forceinline__ int32_t closest_hit(const DevScene& s, f3 o, f3 d, f3
                 cur = cr;
                 continue;
             }
-        } else {
-            uint32_t first, count;
-            if (cur & CODE_SMALL_LEAF) {
-                first = (cur >> 1) & 0x3fffffffu;
-                count = (cur & 1u) + 1u;
-            } else {
-                PtBvhNode n = s.bvh[cur & 0x3fffffffu];
-                first = n.left_first;
-                count = n.tri_count;
+        } else if (cur & CODE_SMALL_LEAF) {
+            // one or two triangles: both are requested before the first test
+            // (a one-triangle leaf loads its triangle twice), then tested in order
+            const uint32_t t0 = (cur >> 1) & 0x3fffffffu, t1 = t0 + (cur & 1u);
+            const PtTriangleGeometry g0 = s.geo[t0], g1 = s.geo[t1];
+            float tt, bv, bw;
+            if (COUNT) ++ctr.tris;
+            if (tri_test(g0.v, o, wt, hit_t, tt, bv, bw) && !passes_through(s, t0, bv, bw, rng)) {
+                hit_t = tt;
+                hit = (int32_t)t0;
+                hit_v = bv;
+                hit_w = bw;
             }
+            if (cur & 1u) {
+                if (COUNT) ++ctr.tris;
+                if (tri_test(g1.v, o, wt, hit_t, tt, bv, bw) && !passes_through(s, t1, bv, bw, rng)) {
+                    hit_t = tt;
+                    hit = (int32_t)t1;
+                    hit_v = bv;
+                    hit_w = bw;
+                }
+            }
+        } else {
+            PtBvhNode n = s.bvh[cur & 0x3fffffffu];
+            const uint32_t first = n.left_first, count = n.tri_count;
             for (uint32_t i = 0; i < count; ++i) {
                 uint32_t ti = first + i;
                 if (COUNT) ++ctr.tris;
@@ -455,16 +470,28 @@ __device__ __forceinline__ bool occluded(const DevScene& s, f3 o, f3 d, float hi
                 cur = cr;
                 continue;
             }
-        } else {
-            uint32_t first, count;
-            if (cur & CODE_SMALL_LEAF) {
-                first = (cur >> 1) & 0x3fffffffu;
-                count = (cur & 1u) + 1u;
-            } else {
-                PtBvhNode n = s.bvh[cur & 0x3fffffffu];
-                first = n.left_first;
-                count = n.tri_count;
+        } else if (cur & CODE_SMALL_LEAF) {
+            // as in closest_hit: both triangles requested, then tested in order
+            const uint32_t t0 = (cur >> 1) & 0x3fffffffu, t1 = t0 + (cur & 1u);
+            const PtTriangleGeometry g0 = s.geo[t0], g1 = s.geo[t1];
+            float tt, bv, bw;
+            if ((int32_t)t0 != light_tri) {
+                if (COUNT) ++ctr.tris;
+                if (tri_test(g0.v, o, wt, hit_t, tt, bv, bw) && !passes_through(s, t0, bv, bw, rng)) {
+                    st.sp = 0;
+                    return true;
+                }
             }
+            if ((cur & 1u) && (int32_t)t1 != light_tri) {
+                if (COUNT) ++ctr.tris;
+                if (tri_test(g1.v, o, wt, hit_t, tt, bv, bw) && !passes_through(s, t1, bv, bw, rng)) {
+                    st.sp = 0;
+                    return true;
+                }
+            }
+        } else {
+            PtBvhNode n = s.bvh[cur & 0x3fffffffu];
+            const uint32_t first = n.left_first, count = n.tri_count;
             for (uint32_t i = 0; i < count; ++i) {
                 uint32_t ti = first + i;
                 if ((int32_t)ti == light_tri) continue;
