@@ -175,9 +175,10 @@ def f32vec(*xs):
 _koh = None
 
 
-def kernel_on_host(desc, image_w, image_h, max_bounces, sample, tile, origin=(0, 0)):
+def kernel_on_host(desc, image_w, image_h, max_bounces, sample, tile, origin=(0, 0), pair_leaves=False):
     """One sample (1-based index `sample`) of a tile by the device code of
-    pt_device.h run on the CPU; returns the per-pixel path radiance (h, w, 3)."""
+    pt_device.h run on the CPU; returns the per-pixel path radiance (h, w, 3).
+    pair_leaves selects the traversal's paired triangle loads for small leaves."""
     global _koh
     if _koh is None:
         path = os.path.join(_HERE, "libkernel_on_host.so")
@@ -185,9 +186,9 @@ def kernel_on_host(desc, image_w, image_h, max_bounces, sample, tile, origin=(0,
             raise RuntimeError(f"{path} not built (make -C oracle)")
         _koh = C.CDLL(path)
         _koh.kernel_on_host_render.restype = C.c_int
-        _koh.kernel_on_host_render.argtypes = [C.POINTER(ptcore.SceneDesc)] + [C.c_uint32] * 8 + [_p]
+        _koh.kernel_on_host_render.argtypes = [C.POINTER(ptcore.SceneDesc)] + [C.c_uint32] * 9 + [_p]
     tw, th = tile
     out = np.zeros((th, tw, 3), np.float32)
     assert _koh.kernel_on_host_render(C.byref(desc), image_w, image_h, origin[0], origin[1], tw, th, max_bounces,
-                                      sample, _ptr(out)) == 0
+                                      sample, 1 if pair_leaves else 0, _ptr(out)) == 0
     return out

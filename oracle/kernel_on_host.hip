@@ -36,7 +36,7 @@ void srgb_table(float lut[256]) {  // IEC 61966-2-1 EOTF, as the library's conte
 
 extern "C" int kernel_on_host_render(const pt_scene_desc* d, uint32_t image_w, uint32_t image_h, uint32_t origin_x,
                                      uint32_t origin_y, uint32_t tile_w, uint32_t tile_h, uint32_t max_bounces,
-                                     uint32_t sample, float* out_rgb) {
+                                     uint32_t sample, uint32_t pair_leaves, float* out_rgb) {
     DevScene s{};
     s.geo = d->geometries;
     s.attr = d->attributes;
@@ -86,8 +86,12 @@ extern "C" int kernel_on_host_render(const pt_scene_desc* d, uint32_t image_w, u
     uint32_t closest = 0, shadow = 0;
     for (uint32_t y = 0; y < tile_h; ++y)
         for (uint32_t x = 0; x < tile_w; ++x) {
-            f3 L = trace_path<false>(s, origin_x + x, origin_y + y, image_w, image_h, max_bounces, sample, st, ctr,
-                                     closest, shadow);
+            // both leaf forms of the traversal: the paired triangle loads of the
+            // traversal kernels, and the plain loop of the whole-path kernels
+            f3 L = pair_leaves ? trace_path<false, true>(s, origin_x + x, origin_y + y, image_w, image_h, max_bounces,
+                                                        sample, st, ctr, closest, shadow)
+                               : trace_path<false, false>(s, origin_x + x, origin_y + y, image_w, image_h, max_bounces,
+                                                         sample, st, ctr, closest, shadow);
             float* o = &out_rgb[((size_t)y * tile_w + x) * 3];
             o[0] = L.x;
             o[1] = L.y;

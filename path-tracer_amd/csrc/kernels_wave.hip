@@ -570,7 +570,7 @@ __global__ __launch_bounds__(WAVE_BLOCK, PT_SHADE_WAVES_PER_EU) void k_shade(Dev
                         const f3 contrib = throughput * f * lwi.z * light.emission * (weight / light.pdf);
                         if (INLINE_SHADOW && !s.all_opaque) {
                             TraversalCounters ctr = {0, 0};
-                            if (!occluded<false>(s, next_origin, light.direction, light.distance, light.triangle_index, rng, st, ctr))
+                            if (!occluded<false, false>(s, next_origin, light.direction, light.distance, light.triangle_index, rng, st, ctr))
                                 L = L + contrib;
                         } else {
                             shadow = true;

@@ -361,7 +361,11 @@ struct TraversalCounters {
 // to the right child, accepts strictly nearer hits, so the hit index equals
 // the reference's on the same BVH. rng is advanced by alpha-tested
 // candidates exactly as passes_through does.
-template <bool COUNT>
+// PAIR: a leaf of one or two triangles loads both before testing them (in
+// order). It pays in the traversal-only kernels; in the register-heavy
+// whole-path kernels (megakernel, inline shadow rays of the shade kernel) the
+// plain loop is faster.
+template <bool COUNT, bool PAIR = true>
 __device__ __forceinline__ int32_t closest_hit(const DevScene& s, f3 o, f3 d, f3 rcp, Rng& rng, Stack& st,
                                                float& hit_t, float& hit_v, float& hit_w, TraversalCounters& ctr) {
     const Watertight wt = watertight_setup(d, rcp);
@@ -395,7 +399,7 @@ __device__ __forceinline__ int32_t closest_hit(const DevScene& s, f3 o, f3 d, f3
                 cur = cr;
                 continue;
             }
-        } else if (cur & CODE_SMALL_LEAF) {
+        } else if (PAIR && (cur & CODE_SMALL_LEAF)) {
             // one or two triangles: both are requested before the first test
             // (a one-triangle leaf loads its triangle twice), then tested in order
             const uint32_t t0 = (cur >> 1) & 0x3fffffffu, t1 = t0 + (cur & 1u);
@@ -418,8 +422,15 @@ __device__ __forceinline__ int32_t closest_hit(const DevScene& s, f3 o, f3 d, f3
                 }
             }
         } else {
-            PtBvhNode n = s.bvh[cur & 0x3fffffffu];
-            const uint32_t first = n.left_first, count = n.tri_count;
+            uint32_t first, count;
+            if (cur & CODE_SMALL_LEAF) {
+                first = (cur >> 1) & 0x3fffffffu;
+                count = (cur & 1u) + 1u;
+            } else {
+                PtBvhNode n = s.bvh[cur & 0x3fffffffu];
+                first = n.left_first;
+                count = n.tri_count;
+            }
             for (uint32_t i = 0; i < count; ++i) {
                 uint32_t ti = first + i;
                 if (COUNT) ++ctr.tris;
@@ -440,7 +451,7 @@ __device__ __forceinline__ int32_t closest_hit(const DevScene& s, f3 o, f3 d, f3
 }
 
 // Any hit (trace.slang:614-717), skipping the sampled light triangle.
-template <bool COUNT>
+template <bool COUNT, bool PAIR = true>
 __device__ __forceinline__ bool occluded(const DevScene& s, f3 o, f3 d, float hit_t, int32_t light_tri, Rng& rng,
                                          Stack& st, TraversalCounters& ctr) {
     const f3 rcp = f3{1.0f, 1.0f, 1.0f} / d;
@@ -470,7 +481,7 @@ __device__ __forceinline__ bool occluded(const DevScene& s, f3 o, f3 d, float hi
                 cur = cr;
                 continue;
             }
-        } else if (cur & CODE_SMALL_LEAF) {
+        } else if (PAIR && (cur & CODE_SMALL_LEAF)) {
             // as in closest_hit: both triangles requested, then tested in order
             const uint32_t t0 = (cur >> 1) & 0x3fffffffu, t1 = t0 + (cur & 1u);
             const PtTriangleGeometry g0 = s.geo[t0], g1 = s.geo[t1];
@@ -490,8 +501,15 @@ __device__ __forceinline__ bool occluded(const DevScene& s, f3 o, f3 d, float hi
                 }
             }
         } else {
-            PtBvhNode n = s.bvh[cur & 0x3fffffffu];
-            const uint32_t first = n.left_first, count = n.tri_count;
+            uint32_t first, count;
+            if (cur & CODE_SMALL_LEAF) {
+                first = (cur >> 1) & 0x3fffffffu;
+                count = (cur & 1u) + 1u;
+            } else {
+                PtBvhNode n = s.bvh[cur & 0x3fffffffu];
+                first = n.left_first;
+                count = n.tri_count;
+            }
             for (uint32_t i = 0; i < count; ++i) {
                 uint32_t ti = first + i;
                 if ((int32_t)ti == light_tri) continue;
@@ -943,7 +961,7 @@ __device__ __forceinline__ f3 sample_bsdf(const Bsdf& b, f3 wo, Rng& rng) {
 
 // ------------------------------------------------------------------ whole path (the megakernel's per-thread loop)
 // One path sample (trace.slang:726-1297), returning final_color.
-template <bool COUNT>
+template <bool COUNT, bool PAIR = false>
 __device__ __forceinline__ f3 trace_path(const DevScene& s, uint32_t px, uint32_t py, uint32_t iw, uint32_t ih,
                                          uint32_t max_bounces, uint32_t sample_count, Stack& st,
                                          TraversalCounters& ctr, uint32_t& closest_rays, uint32_t& shadow_rays) {
@@ -958,7 +976,7 @@ __device__ __forceinline__ f3 trace_path(const DevScene& s, uint32_t px, uint32_
     for (uint32_t bounce = 0; bounce <= max_bounces; ++bounce) {
         float hit_t, hit_v, hit_w;
         if (COUNT) ++closest_rays;
-        int32_t hit = closest_hit<COUNT>(s, ray_origin, ray_direction, rcp, rng, st, hit_t, hit_v, hit_w, ctr);
+        int32_t hit = closest_hit<COUNT, PAIR>(s, ray_origin, ray_direction, rcp, rng, st, hit_t, hit_v, hit_w, ctr);
         if (hit == -1) {
             float mis = 1.0f;
             if (bounce > 0u) mis = power_heuristic(previous_bsdf_pdf, skybox_light_pdf(s, ray_direction, prob_sky));
@@ -980,7 +998,7 @@ __device__ __forceinline__ f3 trace_path(const DevScene& s, uint32_t px, uint32_
             float pdf;
             evaluate_lobes(b, sf.wo, lwi, f, pdf);
             if (COUNT) ++shadow_rays;
-            if (!occluded<COUNT>(s, next_origin, light.direction, light.distance, light.triangle_index, rng, st, ctr)) {
+            if (!occluded<COUNT, PAIR>(s, next_origin, light.direction, light.distance, light.triangle_index, rng, st, ctr)) {
                 float weight = bounce < max_bounces ? power_heuristic(light.pdf, pdf) : 1.0f;
                 final_color = final_color + throughput * f * lwi.z * light.emission * (weight / light.pdf);
             }

@@ -15,11 +15,13 @@ import ptcore
 
 
 def check(desc, image_w, image_h, max_bounces, sample, tile, origin):
-    got = oracle.kernel_on_host(desc, image_w, image_h, max_bounces, sample, tile, origin)
     ref = oracle.render_f32(desc, image_w, image_h, max_bounces, 1, first_sample=sample, tile=tile, origin=origin)[0]
-    diff = np.any(got != ref[..., :3], axis=-1)
-    assert not diff.any(), f"{int(diff.sum())} pixels differ, first at {np.argwhere(diff)[:3].tolist()}"
-    assert np.isfinite(got).all() and got.max() > 0
+    # both leaf forms of the traversal (paired triangle loads, plain loop)
+    for pair in (False, True):
+        got = oracle.kernel_on_host(desc, image_w, image_h, max_bounces, sample, tile, origin, pair_leaves=pair)
+        diff = np.any(got != ref[..., :3], axis=-1)
+        assert not diff.any(), f"pair={pair}: {int(diff.sum())} pixels differ, first at {np.argwhere(diff)[:3].tolist()}"
+        assert np.isfinite(got).all() and got.max() > 0
 
 
 @pytest.mark.parametrize("sample,max_bounces", [(1, 8), (5, 3)])
