@@ -42,6 +42,10 @@ constexpr uint32_t WAVE_BLOCK = 256;
 #ifndef PT_SHADE_WINDOW
 #define PT_SHADE_WINDOW 2u
 #endif
+// smallest render (paths) that two lanes share when it fits one or two chunks
+#ifndef PT_LANE_SPLIT_PATHS
+#define PT_LANE_SPLIT_PATHS (1u << 21)
+#endif
 // blocks per CU of the shade launches
 #ifndef PT_SHADE_BLOCKS_PER_CU
 #define PT_SHADE_BLOCKS_PER_CU 32u
@@ -977,7 +981,7 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
     // automatically for a render of three chunks or more, or of at least
     // LANE_SPLIT_PATHS paths in one or two chunks, which is then cut into two
     // equal chunks (a 1080p 8-spp job: 2.14 -> 2.20 G path samples/s).
-    constexpr uint64_t LANE_SPLIT_PATHS = 1u << 21;
+    constexpr uint64_t LANE_SPLIT_PATHS = PT_LANE_SPLIT_PATHS;
     const bool split = total_paths <= 2ull * chunk_cap &&
                        (cfg.streams == 2 || (cfg.streams == 0 && total_paths >= LANE_SPLIT_PATHS));
     const uint32_t lanes = cfg.streams == 2 || (cfg.streams == 0 && (split || total_paths > 2ull * chunk_cap)) ? 2u : 1u;
