@@ -169,22 +169,34 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_generate(DevScene s0, WaveJob j0
     const uint32_t n = units.n_paths;
     const uint32_t stride = gridDim.x * WAVE_BLOCK;
     if (units.all_active) {
-        // one job's consecutive blocks, every pixel in the tile and image: the
-        // unit of path i is i / unit_paths, and the queue is 0 .. n-1 in order
-        // (what the appends below produce when no path is inactive)
-        const WaveJob& jb = job_of<MULTI>(j0, jobs, units.job0);
-        const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, units.job0);
+        // every pixel of every unit is in its tile and image: the queue is
+        // 0 .. n-1 in order (what the appends below produce when no path is
+        // inactive, in a scattered order). For one job's consecutive blocks the
+        // unit of path i is i / unit_paths and no table is read.
         for (uint32_t i = blockIdx.x * WAVE_BLOCK + threadIdx.x; i < n; i += stride) {
-            const uint32_t u = i / units.unit_paths, rel = i - u * units.unit_paths;
+            uint32_t ji, block, rel;
+            if (units.contiguous) {
+                const uint32_t u = i / units.unit_paths;
+                ji = units.job0;
+                block = units.block0 + u;
+                rel = i - u * units.unit_paths;
+            } else {
+                const uint32_t u = find_unit(units, i);
+                ji = units.job[u];
+                block = units.block[u];
+                rel = i - units.offset[u];
+            }
+            const WaveJob& jb = job_of<MULTI>(j0, jobs, ji);
+            const DevScene& s = scene_of<MULTI>(s0, scenes, jobs, ji);
             const uint32_t k = rel >> 6, lane = rel & 63u;
             uint32_t gx, gy;
-            block_pixel((units.block0 + u) * 64u + lane, jb.tile_w, gx, gy);
+            block_pixel(block * 64u + lane, jb.tile_w, gx, gy);
             const uint32_t px = gx + jb.origin_x, py = gy + jb.origin_y;
             Rng rng{ptm::rng_seed(px, py, jb.first_sample + k)};
             f3 d = primary_direction(s, px, py, jb.image_w, jb.image_h, rng);
             st3(ps.dx, ps.dy, ps.dz, i, d);
             ps.rng[i] = rng.s;
-            if (MULTI) ps.job[i] = units.job0;
+            if (MULTI) ps.job[i] = ji;
             queue[i] = i;
         }
         if (blockIdx.x == 0 && threadIdx.x == 0) counts[0] = n;
@@ -1132,14 +1144,25 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
                 break;
             }
         {
+            // a job's blocks are all active when its tile is whole 8x8 blocks inside the image
+            auto whole = [&](uint32_t ji) {
+                const WaveJob& jb = job_tab[ji];
+                return jb.tile_w % 8u == 0 && jb.tile_h % 8u == 0 && (uint64_t)jb.origin_x + jb.tile_w <= jb.image_w &&
+                       (uint64_t)jb.origin_y + jb.tile_h <= jb.image_h;
+            };
             const size_t ub = chunks[c].unit_begin;
             const uint32_t j0i = ujob[ub], b0 = ublock[ub];
-            const WaveJob& jb = job_tab[j0i];
-            bool all = L.units.unit_paths != 0 && jb.tile_w % 8u == 0 && jb.tile_h % 8u == 0 &&
-                       (uint64_t)jb.origin_x + jb.tile_w <= jb.image_w && (uint64_t)jb.origin_y + jb.tile_h <= jb.image_h;
-            for (size_t u = ub; all && u < chunks[c].unit_end; ++u)
-                all = ujob[u] == j0i && ublock[u] == b0 + (uint32_t)(u - ub);
+            bool all = true, contiguous = L.units.unit_paths != 0;
+            uint32_t last_job = UINT32_MAX;
+            for (size_t u = ub; u < chunks[c].unit_end && (all || contiguous); ++u) {
+                if (ujob[u] != last_job) {
+                    last_job = ujob[u];
+                    all = all && whole(last_job);
+                }
+                contiguous = contiguous && ujob[u] == j0i && ublock[u] == b0 + (uint32_t)(u - ub);
+            }
             L.units.all_active = all ? 1u : 0u;
+            L.units.contiguous = all && contiguous ? 1u : 0u;
             L.units.job0 = j0i;
             L.units.block0 = b0;
         }

@@ -53,10 +53,12 @@ struct ChunkUnits {
     uint32_t n_units;
     uint32_t n_paths;
     uint32_t unit_paths;     // paths of every unit when they are all alike (one job, or equal spp), else 0
-    // nonzero when unit u is block block0 + u of job job0 for every u and all
-    // of those blocks' pixels lie in the tile and the image: every path is
-    // then active and the tables need not be read
+    // nonzero when every unit's pixels lie in its job's tile and image: every
+    // path is then active and the bounce-0 queue is 0 .. n_paths-1
     uint32_t all_active;
+    // nonzero when, besides, unit u is block block0 + u of job job0 for every
+    // u, so the unit tables need not be read
+    uint32_t contiguous;
     uint32_t job0, block0;
 };
 
