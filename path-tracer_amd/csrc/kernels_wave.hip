@@ -36,7 +36,7 @@ constexpr uint32_t WAVE_BLOCK = 256;
 #define PT_TRACE_BLOCK 64u
 #endif
 #ifndef PT_TRACE_WAVES_PER_CU
-#define PT_TRACE_WAVES_PER_CU 256u
+#define PT_TRACE_WAVES_PER_CU 512u
 #endif
 // rounds of WAVE_BLOCK queue entries that a shade block partitions into hits and misses together
 #ifndef PT_SHADE_WINDOW
