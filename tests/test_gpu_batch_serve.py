@@ -257,3 +257,54 @@ def test_server_wait_times_out(ctx, scenes):
         srv.wait(waiting)
     finally:
         srv.close()
+
+
+def test_server_concurrent_submit_and_cancel(ctx, scenes):
+    """Threads submit jobs and cancel every other one at once, queued or
+    running. Every job ends done or cancelled, collecting them all returns
+    the budget to zero, and the done jobs' images are those of a lone render."""
+    _, sc = scenes
+    srv = ptcore.Server(ctx, hbm_budget_bytes=1 << 34, max_active_jobs=4, round_paths=1 << 14)
+    submitted, errors = [], []
+    lock = threading.Lock()
+
+    def worker(seed):
+        rng = np.random.default_rng(seed)
+        try:
+            for k in range(8):
+                job = (int(rng.integers(0, len(sc))), 48, 32, int(rng.choice([1, 2, 4])), 3)
+                jid = srv.submit(sc[job[0]], *job[1:])
+                cancel = k % 2 == 1
+                if cancel:
+                    try:
+                        srv.cancel(jid)
+                    except ptcore.PtError as e:
+                        assert e.code == ptcore.PT_ERR_STATE  # it finished first
+                with lock:
+                    submitted.append((jid, job, cancel))
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(s,)) for s in range(3)]
+    try:
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        assert not errors, errors
+        done = []
+        for jid, job, cancel in submitted:
+            try:
+                srv.wait(jid, 60_000)
+                done.append((job, srv.result(jid, job[1], job[2])))
+            except ptcore.PtError as e:
+                assert cancel and e.code == ptcore.PT_ERR_CANCELLED
+                with pytest.raises(ptcore.PtError):
+                    srv.result(jid, job[1], job[2])  # reports the cancellation and releases the record
+        stats = srv.stats()
+    finally:
+        srv.close()
+    assert stats["bytes_in_use"] == 0
+    assert len(done) >= len(submitted) // 2
+    for job, img in done[:4]:
+        assert np.array_equal(img, render_alone(ctx, sc, [job])[0])
