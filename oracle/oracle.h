@@ -67,11 +67,11 @@ float oracle_power_heuristic(float pdf, float other);
 /* evaluate_lobes for a material (base_color rgb, base_weight, metalness,
  * diffuse roughness, specular rgb, specular weight, specular_ior,
  * spec roughness, anisotropy) at wo, wi; E_dielectric given. Writes f[3], pdf. */
-void oracle_evaluate_lobes(const float params[12], const float wo[3], const float wi[3],
+void oracle_evaluate_lobes(const float params[13], const float wo[3], const float wi[3],
                            const float e_dielectric[3], float f_out[3], float* pdf_out);
 /* Sample a BSDF direction like the bounce does (rands r0 = two_pi_rand/2pi,
  * r1, r2); writes wi and returns 1, or 0 if the sample ends the path. */
-int oracle_sample_bsdf(const float params[12], const float wo[3], const float e_dielectric[3],
+int oracle_sample_bsdf(const float params[13], const float wo[3], const float e_dielectric[3],
                        float r0, float r1, float r2, float wi_out[3]);
 /* Sample a light from `origin` with RNG state *rng; writes direction,
  * distance, emission, pdf; returns the triangle index (-1: sky). */

@@ -1013,7 +1013,7 @@ float oracle_power_heuristic(float pdf, float other) {
 
 // params: base rgb(3), base_weight, metalness, diffuse_roughness, spec rgb(3), spec_weight, spec_ior, spec_roughness
 // (anisotropy taken as 0)
-static Kernel<float>::Shading make_shading(const float prm[12], V3<float> wo, const float e[3]) {
+static Kernel<float>::Shading make_shading(const float prm[13], V3<float> wo, const float e[3]) {
     Kernel<float> k(g_empty_scene, g_dummy);
     Kernel<float>::Shading sh;
     sh.base_color = {prm[0], prm[1], prm[2]};
@@ -1022,7 +1022,7 @@ static Kernel<float>::Shading make_shading(const float prm[12], V3<float> wo, co
     sh.base_diffuse_roughness = prm[5];
     sh.specular_color = {prm[6], prm[7], prm[8]};
     sh.specular_weight = prm[9];
-    k.alpha_anisotropic(prm[11], 0.0f, sh.ax, sh.ay);
+    k.alpha_anisotropic(prm[11], prm[12], sh.ax, sh.ay);
     sh.sim = k.specular_ior_modulated(prm[10], sh.specular_weight);
     sh.F0_conductor = sh.base_weight * sh.base_color;
     sh.one_plus_lambda_wo = 1.0f + k.ggx_lambda(wo, sh.ax, sh.ay);
@@ -1052,7 +1052,7 @@ static Kernel<float>::Shading make_shading(const float prm[12], V3<float> wo, co
     return sh;
 }
 
-void oracle_evaluate_lobes(const float prm[12], const float wo[3], const float wi[3], const float e[3], float f_out[3],
+void oracle_evaluate_lobes(const float prm[13], const float wo[3], const float wi[3], const float e[3], float f_out[3],
                            float* pdf_out) {
     Kernel<float> k(g_empty_scene, g_dummy);
     V3<float> w = {wo[0], wo[1], wo[2]};
@@ -1066,7 +1066,7 @@ void oracle_evaluate_lobes(const float prm[12], const float wo[3], const float w
     *pdf_out = pdf;
 }
 
-int oracle_sample_bsdf(const float prm[12], const float wo[3], const float e[3], float r0, float r1, float r2,
+int oracle_sample_bsdf(const float prm[13], const float wo[3], const float e[3], float r0, float r1, float r2,
                        float wi_out[3]) {
     Kernel<float> k(g_empty_scene, g_dummy);
     V3<float> w = {wo[0], wo[1], wo[2]};

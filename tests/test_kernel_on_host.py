@@ -48,3 +48,10 @@ def test_textured_ingested_scene():
     glb, sky = scene_files()
     w = ptcore.Assets(glb, sky).world(skybox_luminance=2.0, camera_position=(0.0, 1.2, 3.5), pitch=-12.0, fov=50.0)
     check(w.desc, 128, 128, 6, 1, (64, 64), (32, 32))
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_random_openpbr_materials(seed):
+    from materials import random_openpbr_world
+    w = random_openpbr_world(seed)
+    check(w.desc, 128, 128, 6, seed, (64, 64), (32, 32))
