@@ -1,6 +1,7 @@
 """Scenes whose materials take the OpenPBR parameters across their ranges:
 anisotropic GGX, partial metalness and specular weight, coloured specular,
-varied IOR, and tangent-space normal maps at several scales, on the Cornell
+varied IOR, tangent-space normal maps at several scales, and specular colour,
+specular weight and emission textures, on the Cornell
 box geometry. Shared by the CPU test of the kernel source and the GPU parity
 tests.
 """
@@ -41,10 +42,22 @@ def random_openpbr_world(seed, image_w=128, image_h=128, atlas_size=16):
     for k in ("t0", "t1", "t2"):
         att[k][:, :3] = rng.normal(size=(len(att), 3)).astype(np.float32)
     att["t0"][:, 3] = rng.choice(np.array([-1.0, 1.0], dtype=np.float32), len(att))
-    # normals mostly toward +z in tangent space, some strongly tilted
-    lin = np.empty((1, atlas_size, atlas_size, 4), dtype=np.uint8)
-    lin[..., 0:2] = rng.integers(0, 256, (atlas_size, atlas_size, 2))
-    lin[..., 2] = rng.integers(100, 256, (atlas_size, atlas_size))
-    lin[..., 3] = 255
-    inp = ptcore.WorldInput(geo, att, mat, a["skybox"], linear=lin, atlas_size=atlas_size)
+    # specular colour (sRGB layer 0) and specular weight (linear layer 1, alpha) textures
+    full = [0.0, 0.0, 1.0, 1.0]
+    coloured = surf[rng.uniform(size=n) < 0.5]
+    mat["specular_color_tex_layer"][coloured] = 0
+    mat["specular_color_uv"][coloured] = full
+    weighted = surf[rng.uniform(size=n) < 0.5]
+    mat["specular_weight_tex_layer"][weighted] = 1
+    mat["specular_weight_uv"][weighted] = full
+    # the emitter's colour comes from sRGB layer 1 (trace.slang:455)
+    lights = np.setdiff1d(np.arange(len(mat)), surf)
+    mat["emission_tex_layer"][lights] = 1
+    mat["emission_uv"][lights] = full
+    srgb = rng.integers(40, 256, (2, atlas_size, atlas_size, 4), dtype=np.uint8)
+    # layer 0: normals mostly toward +z in tangent space, some strongly tilted; layer 1: random
+    lin = rng.integers(0, 256, (2, atlas_size, atlas_size, 4), dtype=np.uint8)
+    lin[0, ..., 2] = rng.integers(100, 256, (atlas_size, atlas_size))
+    lin[0, ..., 3] = 255
+    inp = ptcore.WorldInput(geo, att, mat, a["skybox"], srgb=srgb, linear=lin, atlas_size=atlas_size)
     return inp.build()
