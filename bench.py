@@ -236,7 +236,7 @@ def bench_serve(args):
         for jid, res, spp in ids:
             srv.wait(jid, 600_000)
             info = srv.info(jid)
-            srv.result(jid, res, res)
+            srv.result(jid)
             lat.append((info["done_s"] - info["submit_s"]) * 1e3)
             first = info["submit_s"] if first is None else min(first, info["submit_s"])
             last = max(last, info["done_s"])

@@ -187,13 +187,17 @@ typedef struct pt_scene pt_scene;
 
 /* Copy a scene to device memory owned by the library (WorldResources::new). */
 pt_status pt_scene_create(pt_context* ctx, const pt_scene_desc* desc, pt_scene** out);
+/* PT_ERR_STATE (scene kept) while a render server has jobs over the scene
+ * that are not done, failed or cancelled. */
 pt_status pt_scene_destroy(pt_scene* scene);
 /* Device bytes held by the scene (used for admission by HBM footprint). */
 pt_status pt_scene_footprint_bytes(const pt_scene* scene, uint64_t* bytes);
 
 /* Partial updates (WorldResources::apply_updates, world_resources.rs:479-525).
  * Each one bumps the scene's version so tracers reset their accumulation
- * (path_tracer.rs:272-282). Arrays are borrowed for the call. */
+ * (path_tracer.rs:272-282). Arrays are borrowed for the call. While server
+ * jobs over the scene are unfinished, they return PT_ERR_STATE and change
+ * nothing, so a job never mixes two versions of a scene. */
 pt_status pt_scene_update_camera(pt_scene* scene, const PtCamera* camera);
 pt_status pt_scene_update_materials(pt_scene* scene, const uint32_t* indices,
                                     const PtMaterial* materials, uint32_t count);
@@ -256,6 +260,12 @@ pt_status pt_tracer_read_storage(pt_tracer* tracer, float* rgba);
 /* PathTracer::display + readback: ACES filmic tone map and sRGB encode of the
  * storage texture into RGBA8 (display.wgsl:19-46). */
 pt_status pt_tracer_display_rgba8(pt_tracer* tracer, uint8_t* rgba);
+/* The display pass on host data (display.wgsl:19-46): the ACES fit and the
+ * sRGB encode of `count` float4 texels (rgba, count * 4 floats) into RGBA8,
+ * rounded to nearest as an Rgba8Unorm target stores them (alpha clamped to
+ * [0, 1]). The same kernel as pt_tracer_display_rgba8, on the context's stream;
+ * synchronises. */
+pt_status pt_tonemap_rgba8(pt_context* ctx, const float* rgba, uint32_t count, uint8_t* out);
 /* Primary-ray closest hits of the tile for 1-based sample index
  * `sample_count` (same jitter as the trace): triangle index (-1 on miss), t. */
 pt_status pt_tracer_primary_hits(pt_tracer* tracer, uint32_t sample_count, int32_t* tri, float* t);
@@ -300,7 +310,14 @@ typedef struct pt_wave_config {
                                      once fewer than half of the wave is still traversing) */
     uint32_t streams;          /* 1, or 2: chunks alternate between two HIP streams with their own
                                   path state, so one chunk's shading overlaps the other's traversal;
-                                  0: automatic (2 when a render spans three chunks or more) */
+                                  0: automatic (2 when a render spans three chunks or more, or is at
+                                  least lane_split_paths and fits two chunks) */
+    uint32_t trace_waves_per_cu;  /* traversal grid: one-wave blocks per CU, 16..2048 (default 512) */
+    uint32_t shade_blocks_per_cu; /* shade grid: 256-thread blocks per CU, 1..256 (default 32) */
+    uint32_t shade_window;        /* queue rounds of 256 entries that a shade block splits into
+                                     surface hits and misses before shading: 1, 2 or 4 (default 2) */
+    uint32_t lane_split_paths;    /* automatic lanes: a render of at least this many paths that fits
+                                     two chunks is cut into two chunks on two lanes (default 2^21) */
 } pt_wave_config;
 #define PT_TRAVERSAL_BATCH 0
 #define PT_TRAVERSAL_REFILL 1
@@ -318,10 +335,14 @@ pt_status pt_context_get_wave_config(const pt_context* ctx, pt_wave_config* conf
 pt_status pt_trace_batch(pt_tracer* const* tracers, uint32_t count, const uint32_t* samples);
 
 /* Autotuner over the batch: times it under wave configurations (coordinate
- * descent over LDS stack depth {16, 24, 32}, chunk size {1M, 2M, 4M, 8M}
- * paths and sort key {none, octant, material}), keeps the fastest in the
- * context and reports it. Every trial restarts the tracers at sample 1; they
- * are left reset (sample_count 0). */
+ * descent over the traversal mode, chunk size {2M, 4M, 8M, 16M} paths, sort
+ * key {none, octant, material}, LDS stack depth {16, 24, 32}, lanes {auto, 1,
+ * 2}, traversal grid {256, 512, 1024} waves per CU, shade grid {16, 32, 64}
+ * blocks per CU and shade window {1, 2, 4}), keeps the fastest in the context
+ * and reports it. A candidate that runs out of device memory is skipped; on
+ * any other error the context keeps the best configuration found so far.
+ * Every trial restarts the tracers at sample 1; they are left reset
+ * (sample_count 0). */
 typedef struct pt_tune_result {
     pt_wave_config best;
     float best_ms;      /* batch time under the best configuration */
@@ -333,17 +354,30 @@ pt_status pt_autotune_batch(pt_tracer* const* tracers, uint32_t count, const uin
 /* ------------------------------------------------------------------ serving */
 /* A render server on a context: a worker thread that keeps scenes resident,
  * admits queued jobs while slots and the HBM budget allow, and traces them
- * with continuous batching: each round is one wavefront chunk made of pixel
- * blocks from every active job, and jobs submitted meanwhile join the next
- * round. A job's pixels always take all of their samples within one round,
- * in sample order, so its image is bit-identical however it is mixed. */
+ * with continuous batching. Each round is one wavefront chunk made of pixel
+ * blocks from every active job; jobs submitted meanwhile join the next
+ * round. Rounds are pipelined over two lanes (HIP streams with their own
+ * path state): while one round runs, the next is built and launched on the
+ * other lane, so a round's last, small bounces overlap the next round's
+ * first, large ones. A job's pixels always take all of their samples within
+ * one round, in sample order, so its image is bit-identical however it is
+ * mixed. Job buffers are sub-allocated from an arena reserved when the
+ * server is created: admission, completion and collection allocate and free
+ * no device memory.
+ *
+ * A scene stays in use from the submission of a job over it until that job
+ * is done, failed or cancelled: pt_scene_destroy and the pt_scene_update_*
+ * calls return PT_ERR_STATE for it meanwhile. */
 typedef struct pt_server pt_server;
 typedef struct pt_server_config {
-    uint64_t hbm_budget_bytes;  /* admission budget: resident scenes + job buffers, + the worker's
-                                   path state while another job runs (a job that fits alone
+    uint64_t hbm_budget_bytes;  /* admission budget: resident scenes + admitted jobs' buffers, + the
+                                   worker's path state while another job runs (a job that fits alone
                                    always runs) */
     uint32_t max_active_jobs;   /* job slots */
     uint32_t round_paths;       /* paths per round (0: the context's chunk size) */
+    uint64_t job_arena_bytes;   /* device arena for job buffers, reserved at creation
+                                   (0: min(hbm_budget_bytes, 4 GiB)); a request whose buffer
+                                   exceeds it is rejected */
 } pt_server_config;
 typedef struct pt_render_request {
     pt_scene* scene;            /* a scene of the server's context */
@@ -363,32 +397,40 @@ typedef struct pt_job_info {
     double admit_s;
     double done_s;              /* result ready (0 until done) */
     uint64_t footprint_bytes;   /* device bytes held by the job */
+    uint64_t result_bytes;      /* bytes pt_server_result writes: width * height * 16 */
 } pt_job_info;
 typedef struct pt_server_stats {
     uint64_t rounds;
     uint64_t paths;             /* path samples traced */
     uint64_t jobs_done;
     uint64_t jobs_rejected;
-    double busy_s;              /* time the worker spent in rounds */
+    double busy_s;              /* time with at least one round on the device */
     uint64_t bytes_in_use;      /* admitted footprint now */
+    uint32_t max_jobs_per_round;    /* most jobs that shared one round */
+    uint32_t max_rounds_in_flight;  /* most rounds on the device at once (2 when lanes overlapped) */
+    uint64_t arena_bytes;       /* size of the job arena */
 } pt_server_stats;
 pt_status pt_server_create(pt_context* ctx, const pt_server_config* config, pt_server** out);
-/* Cancels queued jobs, finishes the running round, joins the worker. */
+/* Cancels queued jobs, finishes the rounds in flight, joins the worker. */
 pt_status pt_server_destroy(pt_server* server);
-/* Queue a job; PT_ERR_MEMORY if it can never fit the budget. Thread-safe. */
+/* Queue a job; PT_ERR_MEMORY if it can never fit the budget or the arena. Thread-safe. */
 pt_status pt_server_submit(pt_server* server, const pt_render_request* request, uint64_t* job_id);
-/* Wait until the job is done, failed or cancelled; PT_ERR_TIMEOUT after timeout_ms. */
+/* Wait until the job is done, failed or cancelled; PT_ERR_TIMEOUT after
+ * timeout_ms; PT_ERR_INVALID if the job is unknown or was collected
+ * meanwhile by another thread. */
 pt_status pt_server_wait(pt_server* server, uint64_t job_id, uint32_t timeout_ms);
 pt_status pt_server_info(pt_server* server, uint64_t job_id, pt_job_info* info);
 /* Copy a done job's accumulation (width*height float4, the sum of its
- * samples) to host memory and release the job. For a failed or cancelled
- * job, return PT_ERR_DEVICE or PT_ERR_CANCELLED and release it (rgba is not
- * written); PT_ERR_STATE while it is queued or running. */
-pt_status pt_server_result(pt_server* server, uint64_t job_id, float* rgba);
+ * samples) to host memory and release the job. `capacity` is the size of
+ * `rgba` in bytes; PT_ERR_INVALID (job kept) when it is smaller than the
+ * job's result_bytes. For a failed or cancelled job, return PT_ERR_DEVICE or
+ * PT_ERR_CANCELLED and release it (rgba is not written); PT_ERR_STATE while
+ * it is queued or running. */
+pt_status pt_server_result(pt_server* server, uint64_t job_id, float* rgba, size_t capacity);
 /* Cancel a job. A queued job is cancelled at once. A running job is taken
- * out of scheduling and cancelled, with its buffer freed, once the round in
- * flight ends; if that round completes it, it is done instead. PT_ERR_STATE
- * for a finished job. */
+ * out of scheduling and cancelled, with its buffer released, once its rounds
+ * in flight end; if they complete it, it is done instead. PT_ERR_STATE for a
+ * finished job. */
 pt_status pt_server_cancel(pt_server* server, uint64_t job_id);
 pt_status pt_server_stats_get(pt_server* server, pt_server_stats* out);
 

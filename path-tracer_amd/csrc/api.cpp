@@ -73,6 +73,12 @@ void srgb_table(float out[256]) {
     }
 }
 
+pt_status scene_require_idle(const Scene* s, const char* what) {
+    const uint32_t n = s->server_jobs.load();
+    if (n) return set_error(PT_ERR_STATE, "%s: %u render-server job(s) over this scene are not finished", what, n);
+    return PT_OK;
+}
+
 pt_status Context::ensure_overflow(uint32_t depth, uint32_t grid_threads) {
     uint32_t extra = depth > ptd::MEGA_LDS_DEPTH ? depth - ptd::MEGA_LDS_DEPTH : 1u;
     size_t need = (size_t)extra * grid_threads * sizeof(uint32_t);
@@ -357,6 +363,7 @@ pt_status pt_scene_create(pt_context* c, const pt_scene_desc* d, pt_scene** out)
 
 pt_status pt_scene_destroy(pt_scene* s) {
     if (!s) return PT_OK;
+    if (pt_status st = scene_require_idle(s, "pt_scene_destroy")) return st;
     (void)hipSetDevice(s->ctx->device);
     const hipError_t pending = hipStreamSynchronize(s->ctx->stream);
     const ptd::DevScene& v = s->view;
@@ -383,6 +390,7 @@ pt_status pt_scene_footprint_bytes(const pt_scene* s, uint64_t* bytes) {
 
 pt_status pt_scene_update_camera(pt_scene* s, const PtCamera* camera) {
     if (!s || !camera) return set_error(PT_ERR_INVALID, "null argument");
+    if (pt_status st = scene_require_idle(s, "pt_scene_update_camera")) return st;
     s->view.camera = *camera;
     ++s->version;
     return PT_OK;
@@ -390,6 +398,7 @@ pt_status pt_scene_update_camera(pt_scene* s, const PtCamera* camera) {
 
 pt_status pt_scene_update_materials(pt_scene* s, const uint32_t* idx, const PtMaterial* m, uint32_t n) {
     if (!s || (n && (!idx || !m))) return set_error(PT_ERR_INVALID, "null argument");
+    if (pt_status st = scene_require_idle(s, "pt_scene_update_materials")) return st;
     for (uint32_t i = 0; i < n; ++i) {
         if (idx[i] >= s->material_count) return set_error(PT_ERR_INVALID, "material index %u out of range", idx[i]);
         if (m[i].alpha_mode != PT_ALPHA_MODE_OPAQUE && s->view.all_opaque)
@@ -444,6 +453,7 @@ static void set_skybox_uniforms(pt_scene* s, const PtSkyboxUniforms& sky) {
 pt_status pt_scene_update_lights(pt_scene* s, const PtSceneUniforms* u, const PtSceneAliasEntry* alias, uint32_t n,
                                  const PtSkyboxUniforms* sky) {
     if (!s || !u || (n && !alias)) return set_error(PT_ERR_INVALID, "null argument");
+    if (pt_status ist = scene_require_idle(s, "pt_scene_update_lights")) return ist;
     pt_status st = validate_lights(s, u, alias, n);
     if (st) return st;
     if (pt_status dst = use_device(s->ctx->device)) return dst;
@@ -457,6 +467,7 @@ pt_status pt_scene_update_lights(pt_scene* s, const PtSceneUniforms* u, const Pt
 
 pt_status pt_scene_apply_updates(pt_scene* s, const pt_world_updates* u) {
     if (!s || !u) return set_error(PT_ERR_INVALID, "null argument");
+    if (pt_status st = scene_require_idle(s, "pt_scene_apply_updates")) return st;
     if (u->material_count && (!u->material_indices || !u->materials))
         return set_error(PT_ERR_INVALID, "material updates without arrays");
     if (u->scene_alias_changed && u->scene_alias_count && !u->scene_alias)

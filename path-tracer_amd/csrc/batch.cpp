@@ -38,7 +38,42 @@ pt_status check_config(const pt_wave_config* c) {
     if (c->sort_key > PT_SORT_MATERIAL) return set_error(PT_ERR_INVALID, "unknown sort key %u", c->sort_key);
     if (c->traversal > 2) return set_error(PT_ERR_INVALID, "unknown traversal mode %u", c->traversal);
     if (c->streams > 2) return set_error(PT_ERR_INVALID, "streams must be 0 (automatic), 1 or 2");
+    if (c->trace_waves_per_cu < 16 || c->trace_waves_per_cu > 2048)
+        return set_error(PT_ERR_INVALID, "trace_waves_per_cu must be in [16, 2048]");
+    if (c->shade_blocks_per_cu < 1 || c->shade_blocks_per_cu > 256)
+        return set_error(PT_ERR_INVALID, "shade_blocks_per_cu must be in [1, 256]");
+    if (c->shade_window != 1 && c->shade_window != 2 && c->shade_window != 4)
+        return set_error(PT_ERR_INVALID, "shade_window must be 1, 2 or 4");
+    if (c->lane_split_paths == 0) return set_error(PT_ERR_INVALID, "lane_split_paths must be positive");
     return PT_OK;
+}
+
+ptd::WaveConfig from_api(const pt_wave_config& c) {
+    ptd::WaveConfig w;
+    w.lds_depth = c.lds_stack_depth;
+    w.chunk_paths = c.chunk_paths;
+    w.sort_key = c.sort_key;
+    w.traversal = c.traversal;
+    w.streams = c.streams;
+    w.trace_waves_per_cu = c.trace_waves_per_cu;
+    w.shade_blocks_per_cu = c.shade_blocks_per_cu;
+    w.shade_window = c.shade_window;
+    w.lane_split_paths = c.lane_split_paths;
+    return w;
+}
+
+pt_wave_config to_api(const ptd::WaveConfig& w) {
+    pt_wave_config c;
+    c.lds_stack_depth = w.lds_depth;
+    c.chunk_paths = w.chunk_paths;
+    c.sort_key = w.sort_key;
+    c.traversal = w.traversal;
+    c.streams = w.streams;
+    c.trace_waves_per_cu = w.trace_waves_per_cu;
+    c.shade_blocks_per_cu = w.shade_blocks_per_cu;
+    c.shade_window = w.shade_window;
+    c.lane_split_paths = w.lane_split_paths;
+    return c;
 }
 
 pt_status batch_jobs(pt_tracer* const* tracers, uint32_t count, const uint32_t* samples, std::vector<HostJob>& jobs) {
@@ -81,21 +116,13 @@ pt_status pt_context_set_wave_config(pt_context* c, const pt_wave_config* cfg) {
     if (!c || !cfg) return set_error(PT_ERR_INVALID, "null argument");
     pt_status st = check_config(cfg);
     if (st) return st;
-    c->wave_cfg.lds_depth = cfg->lds_stack_depth;
-    c->wave_cfg.chunk_paths = cfg->chunk_paths;
-    c->wave_cfg.sort_key = cfg->sort_key;
-    c->wave_cfg.traversal = cfg->traversal;
-    c->wave_cfg.streams = cfg->streams;
+    c->wave_cfg = from_api(*cfg);
     return PT_OK;
 }
 
 pt_status pt_context_get_wave_config(const pt_context* c, pt_wave_config* cfg) {
     if (!c || !cfg) return set_error(PT_ERR_INVALID, "null argument");
-    cfg->lds_stack_depth = c->wave_cfg.lds_depth;
-    cfg->chunk_paths = c->wave_cfg.chunk_paths;
-    cfg->sort_key = c->wave_cfg.sort_key;
-    cfg->traversal = c->wave_cfg.traversal;
-    cfg->streams = c->wave_cfg.streams;
+    *cfg = to_api(c->wave_cfg);
     return PT_OK;
 }
 
@@ -125,32 +152,40 @@ pt_status pt_autotune_batch(pt_tracer* const* tracers, uint32_t count, const uin
     uint32_t trials = 0;
     pt_status st = trial(best, ms);  // warms allocations, then the timed initial run
     if (st == PT_OK) st = trial(best, best_ms);
-    if (st) return st;
+    if (st) {
+        c->wave_cfg = best;
+        return st;
+    }
     trials = 1;
     out->initial_ms = best_ms;
     // coordinate descent: one knob at a time, keep what is faster
-    const uint32_t lds_opts[] = {16, 24, 32};
-    const uint32_t chunk_opts[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
-    const uint32_t sort_opts[] = {ptd::SORT_NONE, ptd::SORT_OCTANT, ptd::SORT_MATERIAL};
-    const uint32_t traversal_opts[] = {ptd::TRAVERSAL_BATCH, ptd::TRAVERSAL_REFILL, ptd::TRAVERSAL_FETCH};
-    const uint32_t stream_opts[] = {0, 1, 2};
-    for (int knob = 0; knob < 5; ++knob) {
-        const uint32_t* opts = knob == 0   ? traversal_opts
-                               : knob == 1 ? chunk_opts
-                               : knob == 2 ? sort_opts
-                               : knob == 3 ? lds_opts
-                                           : stream_opts;
-        const int nopts = knob == 1 ? 4 : 3;
-        ptd::WaveConfig base = best;
-        for (int k = 0; k < nopts; ++k) {
+    struct Knob {
+        uint32_t ptd::WaveConfig::*field;
+        std::vector<uint32_t> opts;
+    };
+    const Knob knobs[] = {
+        {&ptd::WaveConfig::traversal, {ptd::TRAVERSAL_BATCH, ptd::TRAVERSAL_REFILL, ptd::TRAVERSAL_FETCH}},
+        {&ptd::WaveConfig::chunk_paths, {1u << 21, 1u << 22, 1u << 23, 1u << 24}},
+        {&ptd::WaveConfig::sort_key, {ptd::SORT_NONE, ptd::SORT_OCTANT, ptd::SORT_MATERIAL}},
+        {&ptd::WaveConfig::lds_depth, {16, 24, 32}},
+        {&ptd::WaveConfig::streams, {0, 1, 2}},
+        {&ptd::WaveConfig::trace_waves_per_cu, {256, 512, 1024}},
+        {&ptd::WaveConfig::shade_blocks_per_cu, {16, 32, 64}},
+        {&ptd::WaveConfig::shade_window, {1, 2, 4}},
+    };
+    for (const Knob& knob : knobs) {
+        const ptd::WaveConfig base = best;
+        for (uint32_t v : knob.opts) {
             ptd::WaveConfig cand = base;
-            if (knob == 0) cand.traversal = opts[k];
-            else if (knob == 1) cand.chunk_paths = opts[k];
-            else if (knob == 2) cand.sort_key = opts[k];
-            else if (knob == 3) cand.lds_depth = opts[k];
-            else cand.streams = opts[k];
+            cand.*knob.field = v;
             if (std::memcmp(&cand, &base, sizeof(cand)) == 0) continue;
-            if ((st = trial(cand, ms))) return st;
+            st = trial(cand, ms);
+            if (st == PT_ERR_MEMORY) continue;  // too large for the device now: not a candidate
+            if (st) {
+                c->wave_cfg = best;
+                for (uint32_t i = 0; i < count; ++i) tracers[i]->params.sample_count = 0;
+                return st;
+            }
             ++trials;
             if (ms < best_ms) {
                 best_ms = ms;
@@ -160,11 +195,7 @@ pt_status pt_autotune_batch(pt_tracer* const* tracers, uint32_t count, const uin
     }
     c->wave_cfg = best;
     for (uint32_t i = 0; i < count; ++i) tracers[i]->params.sample_count = 0;
-    out->best.lds_stack_depth = best.lds_depth;
-    out->best.chunk_paths = best.chunk_paths;
-    out->best.sort_key = best.sort_key;
-    out->best.traversal = best.traversal;
-    out->best.streams = best.streams;
+    out->best = to_api(best);
     out->best_ms = best_ms;
     out->trials = trials;
     return PT_OK;
@@ -175,6 +206,57 @@ pt_status pt_autotune_batch(pt_tracer* const* tracers, uint32_t count, const uin
 // ------------------------------------------------------------------ server
 namespace {
 
+// First-fit sub-allocator over one device range reserved at server creation.
+// Blocks are 256-byte aligned; freed blocks merge with free neighbours.
+struct JobArena {
+    char* base = nullptr;
+    uint64_t bytes = 0;
+    std::map<uint64_t, uint64_t> free_blocks;  // offset -> size
+    static uint64_t round_up(uint64_t n) { return (n + 255) & ~(uint64_t)255; }
+    void reset(char* b, uint64_t n) {
+        base = b;
+        bytes = n & ~(uint64_t)255;
+        free_blocks.clear();
+        if (bytes) free_blocks[0] = bytes;
+    }
+    // nullptr when no free block is large enough
+    bool can_take(uint64_t n) const {
+        n = round_up(n);
+        for (const auto& kv : free_blocks)
+            if (kv.second >= n) return true;
+        return false;
+    }
+    void* take(uint64_t n) {
+        n = round_up(n);
+        for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
+            if (it->second < n) continue;
+            const uint64_t off = it->first, size = it->second;
+            free_blocks.erase(it);
+            if (size > n) free_blocks[off + n] = size - n;
+            return base + off;
+        }
+        return nullptr;
+    }
+    void give(void* p, uint64_t n) {
+        if (!p) return;
+        n = round_up(n);
+        uint64_t off = (uint64_t)((char*)p - base);
+        auto next = free_blocks.lower_bound(off);
+        if (next != free_blocks.end() && off + n == next->first) {
+            n += next->second;
+            next = free_blocks.erase(next);
+        }
+        if (next != free_blocks.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += n;
+                return;
+            }
+        }
+        free_blocks[off] = n;
+    }
+};
+
 struct ServerJob {
     uint64_t id;
     pt_render_request req;
@@ -182,29 +264,49 @@ struct ServerJob {
     uint32_t rounds = 0;
     double submit_s = 0, admit_s = 0, done_s = 0;
     uint64_t footprint = 0;
-    float4* accum = nullptr;
+    float4* accum = nullptr;    // in the server's job arena
     uint32_t blocks = 0;        // 8x8 blocks of the image
     uint32_t next_block = 0;    // first block not yet scheduled
-    uint32_t in_flight = 0;     // blocks scheduled in the running round
-    bool cancel = false;        // a running job asked to stop after its round
+    uint32_t in_flight = 0;     // rounds on the device that trace blocks of this job
+    bool cancel = false;        // a running job asked to stop after its rounds in flight
+    bool failed = false;        // a round of the job failed; it ends once no round is in flight
+    bool holds_scene = false;   // counted in req.scene->server_jobs
     std::string error;
 };
+
+// A round on the device: launched on `lane`, complete when `done` has fired.
+struct Round {
+    int lane;
+    std::vector<uint64_t> ids;
+    uint64_t paths;
+    pt_status status;  // of the launches
+};
+
+constexpr int SERVER_LANES = 2;
 
 }  // namespace
 
 struct pt_server {
     Context* ctx = nullptr;
     pt_server_config cfg{};
-    hipStream_t stream = nullptr;
-    Wavefront wave;
+    // two lanes with their own stream and path state: rounds alternate between
+    // them, and the next round is launched while the current one runs
+    hipStream_t lane_stream[SERVER_LANES] = {nullptr, nullptr};
+    hipEvent_t lane_done[SERVER_LANES] = {nullptr, nullptr};
+    bool lane_busy[SERVER_LANES] = {false, false};
+    Wavefront wave[SERVER_LANES];
+    void* arena_mem = nullptr;
+    JobArena arena;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::map<uint64_t, ServerJob> jobs;
     std::deque<uint64_t> queued;
     std::vector<uint64_t> active;
+    std::deque<Round> in_flight;  // oldest first
     uint64_t next_id = 1;
     uint64_t bytes_in_use = 0;  // admitted jobs' buffers
-    uint64_t arena_bytes = 0;   // path state reserved by the worker
+    uint64_t scratch_bytes = 0; // path state reserved by the worker's lanes
+    double busy_since = 0;      // start of the current stretch with rounds in flight
     bool stop = false;
     std::thread worker;
     pt_server_stats stats{};
@@ -231,37 +333,40 @@ struct pt_server {
     bool front_fits() const {
         if (queued.empty() || active.size() >= cfg.max_active_jobs) return false;
         const ServerJob& j = jobs.at(queued.front());
-        const uint64_t scratch = active.empty() ? 0 : arena_bytes;
+        const uint64_t scratch = active.empty() ? 0 : scratch_bytes;
         return bytes_in_use + job_bytes(j.req) + scene_bytes_with(j.req.scene) + scratch <= cfg.hbm_budget_bytes;
     }
 
-    // a job leaves its slot: its buffer is released now, its record stays
-    // until the caller collects it; under mu
+    // the first queued job fits now and the arena has a buffer for it; under mu
+    bool front_admissible() const { return front_fits() && arena.can_take(job_bytes(jobs.at(queued.front()).req)); }
+
+    // the job no longer uses its scene; under mu
+    void release_scene(ServerJob& j) {
+        if (j.holds_scene) {
+            j.req.scene->server_jobs.fetch_sub(1);
+            j.holds_scene = false;
+        }
+    }
+
+    // a job leaves its slot: its buffer returns to the arena now, its record
+    // stays until the caller collects it; under mu, with no round in flight for it
     void finish(ServerJob& j, int32_t state, double t) {
-        if (j.accum) (void)hipFree(j.accum);
+        arena.give(j.accum, j.footprint);
         j.accum = nullptr;
         bytes_in_use -= j.footprint;
         j.footprint = 0;
         j.state = state;
         j.done_s = t;
+        release_scene(j);
     }
 
-    // move queued jobs into slots while the budget allows (FIFO); under mu
+    // move queued jobs into slots while the budget and the arena allow (FIFO); under mu
     void admit() {
-        while (front_fits()) {
+        while (front_admissible()) {
             ServerJob& j = jobs.at(queued.front());
             const uint64_t need = job_bytes(j.req);
-            hipError_t e = hipMalloc(&j.accum, need);
-            if (e != hipSuccess) {
-                j.accum = nullptr;
-                if (!active.empty()) break;  // retry when a running job frees memory
-                // nothing here will free memory: the job cannot start
-                j.error = std::string("device allocation of the job buffer failed: ") + hipGetErrorString(e);
-                finish(j, PT_JOB_FAILED, now_s());
-                queued.pop_front();
-                cv_done.notify_all();
-                continue;
-            }
+            void* buf = arena.take(need);
+            j.accum = (float4*)buf;
             j.footprint = need;
             bytes_in_use += need;
             j.state = PT_JOB_RUNNING;
@@ -272,13 +377,24 @@ struct pt_server {
         }
     }
 
-    // drop finished and cancelled jobs from the slots; under mu
+    // end jobs whose last round came back, and drop finished jobs from the slots; under mu
     void retire() {
         const double t = now_s();
         bool any = false;
         for (uint64_t id : active) {
             ServerJob& j = jobs.at(id);
-            if (j.state == PT_JOB_RUNNING && j.cancel && j.in_flight == 0) {
+            if (j.state != PT_JOB_RUNNING || j.in_flight != 0) continue;
+            if (j.failed) {
+                finish(j, PT_JOB_FAILED, t);
+                any = true;
+            } else if (j.next_block >= j.blocks) {
+                // every block traced and back: the buffer stays until collected
+                j.state = PT_JOB_DONE;
+                j.done_s = t;
+                stats.jobs_done += 1;
+                release_scene(j);
+                any = true;
+            } else if (j.cancel) {
                 finish(j, PT_JOB_CANCELLED, t);
                 any = true;
             }
@@ -289,82 +405,140 @@ struct pt_server {
         if (any) cv_done.notify_all();
     }
 
+    // an active job that retire() would end now (cancelled or failed, nothing in flight); under mu
+    bool retire_pending() const {
+        for (uint64_t id : active) {
+            const ServerJob& j = jobs.at(id);
+            if (j.in_flight == 0 && (j.cancel || j.failed)) return true;
+        }
+        return false;
+    }
+    bool schedulable() const {
+        for (uint64_t id : active) {
+            const ServerJob& j = jobs.at(id);
+            if (!j.cancel && !j.failed && j.next_block < j.blocks) return true;
+        }
+        return false;
+    }
+    int free_lane() const {
+        for (int l = 0; l < SERVER_LANES; ++l)
+            if (!lane_busy[l]) return l;
+        return -1;
+    }
+
+    // blocks from the active jobs in admission order, up to the round size; under mu
+    Round build_round(int lane, std::vector<HostJob>& round) {
+        Round r{lane, {}, 0, PT_OK};
+        const uint32_t round_paths = cfg.round_paths ? cfg.round_paths : ctx->wave_cfg.chunk_paths;
+        for (uint64_t id : active) {
+            ServerJob& j = jobs.at(id);
+            if (j.cancel || j.failed || j.next_block >= j.blocks) continue;
+            const uint64_t per_block = 64ull * j.req.samples;
+            uint64_t room = r.paths >= round_paths ? 0 : (round_paths - r.paths) / per_block;
+            if (room == 0 && !round.empty()) break;
+            room = std::max<uint64_t>(room, 1);
+            const uint32_t take = (uint32_t)std::min<uint64_t>(room, j.blocks - j.next_block);
+            HostJob hj;
+            hj.scene = j.req.scene;
+            hj.job.scene = 0;
+            hj.job.image_w = hj.job.tile_w = j.req.width;
+            hj.job.image_h = hj.job.tile_h = j.req.height;
+            hj.job.origin_x = hj.job.origin_y = 0;
+            hj.job.max_bounces = j.req.max_bounces;
+            hj.job.first_sample = 1;
+            hj.job.samples = j.req.samples;
+            hj.job.accum = j.accum;
+            hj.job.storage = nullptr;
+            hj.block_begin = j.next_block;
+            hj.block_end = j.next_block + take;
+            j.next_block += take;
+            j.in_flight += 1;
+            j.rounds += 1;
+            r.paths += per_block * take;
+            round.push_back(hj);
+            r.ids.push_back(id);
+        }
+        return r;
+    }
+
+    // a round came back (or failed); under mu
+    void complete(const Round& r, bool ok, const std::string& why) {
+        const double t = now_s();
+        stats.rounds += 1;
+        stats.paths += r.paths;
+        lane_busy[r.lane] = false;
+        for (uint64_t id : r.ids) {
+            ServerJob& j = jobs.at(id);
+            j.in_flight -= 1;
+            if (!ok && !j.failed) {
+                j.failed = true;
+                j.error = why;
+            }
+        }
+        if (in_flight.empty()) stats.busy_s += t - busy_since;
+    }
+
     void run() {
         (void)hipSetDevice(ctx->device);  // a failure surfaces as the first round's error
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
-            retire();
-            admit();
-            cv_work.wait(lk, [&] {
-                if (stop) return true;
-                for (uint64_t id : active) {
-                    const ServerJob& j = jobs.at(id);
-                    if (j.cancel || j.next_block < j.blocks) return true;
+            // rounds that came back, oldest first (they may end out of order)
+            for (size_t k = 0; k < in_flight.size();) {
+                Round& r = in_flight[k];
+                hipError_t q = r.status == PT_OK ? hipEventQuery(lane_done[r.lane]) : hipSuccess;
+                if (q == hipErrorNotReady) {
+                    ++k;
+                    continue;
                 }
-                return front_fits();
-            });
-            if (stop) return;
+                Round done = std::move(r);
+                in_flight.erase(in_flight.begin() + (ptrdiff_t)k);
+                std::string why;
+                if (done.status != PT_OK) why = "render failed";
+                else if (q != hipSuccess) why = hipGetErrorString(q);
+                complete(done, why.empty(), why);
+            }
             retire();
             admit();
-            // one round: blocks from the active jobs in admission order
-            const uint32_t round_paths = cfg.round_paths ? cfg.round_paths : ctx->wave_cfg.chunk_paths;
-            std::vector<HostJob> round;
-            std::vector<uint64_t> round_ids;
-            uint64_t paths = 0;
-            for (uint64_t id : active) {
-                ServerJob& j = jobs.at(id);
-                if (j.cancel || j.next_block >= j.blocks) continue;
-                const uint64_t per_block = 64ull * j.req.samples;
-                uint64_t room = paths >= round_paths ? 0 : (round_paths - paths) / per_block;
-                if (room == 0 && !round.empty()) break;
-                room = std::max<uint64_t>(room, 1);
-                const uint32_t take = (uint32_t)std::min<uint64_t>(room, j.blocks - j.next_block);
-                HostJob hj;
-                hj.scene = j.req.scene;
-                hj.job.scene = 0;
-                hj.job.image_w = hj.job.tile_w = j.req.width;
-                hj.job.image_h = hj.job.tile_h = j.req.height;
-                hj.job.origin_x = hj.job.origin_y = 0;
-                hj.job.max_bounces = j.req.max_bounces;
-                hj.job.first_sample = 1;
-                hj.job.samples = j.req.samples;
-                hj.job.accum = j.accum;
-                hj.job.storage = nullptr;
-                hj.block_begin = j.next_block;
-                hj.block_end = j.next_block + take;
-                j.next_block += take;
-                j.in_flight = take;
-                j.rounds += 1;
-                paths += per_block * take;
-                round.push_back(hj);
-                round_ids.push_back(id);
-            }
-            if (round.empty()) continue;
-            lk.unlock();
-            const double t0 = now_s();
-            unsigned long long dummy[6] = {0};
-            pt_status st = wave.render(round, 0, nullptr, dummy);
-            hipError_t e = st == PT_OK ? hipStreamSynchronize(stream) : hipSuccess;
-            const double t1 = now_s();
-            lk.lock();
-            stats.rounds += 1;
-            stats.paths += paths;
-            stats.busy_s += t1 - t0;
-            arena_bytes = wave.arena_bytes + wave.ovf_bytes + wave.tables_bytes;
-            for (uint64_t id : round_ids) {
-                ServerJob& j = jobs.at(id);
-                j.in_flight = 0;
+            if (stop && in_flight.empty()) return;
+            const int lane = stop ? -1 : free_lane();
+            if (lane >= 0 && schedulable()) {
+                std::vector<HostJob> round;
+                Round r = build_round(lane, round);
+                stats.max_jobs_per_round = std::max<uint32_t>(stats.max_jobs_per_round, (uint32_t)round.size());
+                lane_busy[lane] = true;
+                if (in_flight.empty()) busy_since = now_s();
+                lk.unlock();
+                unsigned long long dummy[6] = {0};
+                pt_status st = wave[lane].render(round, 0, nullptr, dummy);
+                hipError_t e = hipSuccess;
+                if (st == PT_OK) e = hipEventRecord(lane_done[lane], lane_stream[lane]);
                 if (st != PT_OK || e != hipSuccess) {
-                    j.error = st != PT_OK ? "render failed" : hipGetErrorString(e);
-                    finish(j, PT_JOB_FAILED, t1);
-                } else if (j.next_block >= j.blocks) {
-                    j.state = PT_JOB_DONE;
-                    j.done_s = t1;
-                    stats.jobs_done += 1;
+                    // some launches may be queued: let them drain before the buffers are reused
+                    (void)hipStreamSynchronize(lane_stream[lane]);
+                    if (st == PT_OK) st = PT_ERR_DEVICE;
                 }
+                lk.lock();
+                r.status = st;
+                in_flight.push_back(std::move(r));
+                stats.max_rounds_in_flight = std::max<uint32_t>(stats.max_rounds_in_flight, (uint32_t)in_flight.size());
+                scratch_bytes = wave[0].device_bytes() + wave[1].device_bytes();
+                continue;  // maybe launch on the other lane too
             }
-            retire();
-            cv_done.notify_all();
+            if (in_flight.empty()) {
+                cv_work.wait(lk, [&] { return stop || schedulable() || front_admissible() || retire_pending(); });
+                continue;
+            }
+            if (lane < 0) {
+                // both lanes busy (or stopping): wait for the oldest round
+                const int l = in_flight.front().lane;
+                lk.unlock();
+                (void)hipEventSynchronize(lane_done[l]);  // its result is read by the query above
+                lk.lock();
+                continue;
+            }
+            // a lane is free but nothing can use it yet: wake for new work or to check the rounds
+            cv_work.wait_for(lk, std::chrono::microseconds(200),
+                             [&] { return stop || schedulable() || front_admissible() || retire_pending(); });
         }
     }
 };
@@ -380,13 +554,28 @@ pt_status pt_server_create(pt_context* c, const pt_server_config* cfg, pt_server
     pt_server* s = new pt_server();
     s->ctx = c;
     s->cfg = *cfg;
-    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        delete s;
-        return hip_error(e, "server stream");
+    uint64_t arena = cfg->job_arena_bytes ? cfg->job_arena_bytes : std::min<uint64_t>(cfg->hbm_budget_bytes, 4ull << 30);
+    arena = std::max<uint64_t>(arena & ~(uint64_t)255, 256);
+    s->cfg.job_arena_bytes = arena;
+    hipError_t e = hipMalloc(&s->arena_mem, arena);
+    for (int l = 0; l < SERVER_LANES && e == hipSuccess; ++l) {
+        e = hipStreamCreateWithFlags(&s->lane_stream[l], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s->lane_done[l], hipEventDisableTiming);
+        s->wave[l].ctx = c;
+        s->wave[l].stream_override = s->lane_stream[l];
+        s->wave[l].one_lane = true;
     }
-    s->wave.ctx = c;
-    s->wave.stream_override = s->stream;
+    if (e != hipSuccess) {
+        for (int l = 0; l < SERVER_LANES; ++l) {
+            if (s->lane_done[l]) (void)hipEventDestroy(s->lane_done[l]);
+            if (s->lane_stream[l]) (void)hipStreamDestroy(s->lane_stream[l]);
+        }
+        if (s->arena_mem) (void)hipFree(s->arena_mem);
+        delete s;
+        return hip_error(e, "server arena and lanes");
+    }
+    s->arena.reset((char*)s->arena_mem, arena);
+    s->stats.arena_bytes = s->arena.bytes;
     s->worker = std::thread([s] { s->run(); });
     *out = s;
     return PT_OK;
@@ -397,18 +586,30 @@ pt_status pt_server_destroy(pt_server* s) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
-        for (uint64_t id : s->queued) s->jobs.at(id).state = PT_JOB_CANCELLED;
+        for (uint64_t id : s->queued) {
+            ServerJob& j = s->jobs.at(id);
+            j.state = PT_JOB_CANCELLED;
+            s->release_scene(j);
+        }
         s->queued.clear();
     }
     s->cv_work.notify_all();
     s->cv_done.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    if (s->worker.joinable()) s->worker.join();  // the worker returns once no round is in flight
     (void)hipSetDevice(s->ctx->device);
-    const hipError_t pending = hipStreamSynchronize(s->stream);
-    for (auto& kv : s->jobs)
-        if (kv.second.accum) (void)hipFree(kv.second.accum);
-    (void)hipStreamDestroy(s->stream);
-    delete s;
+    hipError_t pending = hipSuccess;
+    for (int l = 0; l < SERVER_LANES; ++l) {
+        const hipError_t e = hipStreamSynchronize(s->lane_stream[l]);
+        if (pending == hipSuccess) pending = e;
+    }
+    for (auto& kv : s->jobs) s->release_scene(kv.second);
+    for (int l = 0; l < SERVER_LANES; ++l) {
+        (void)hipEventDestroy(s->lane_done[l]);
+    }
+    (void)hipFree(s->arena_mem);
+    hipStream_t streams[SERVER_LANES] = {s->lane_stream[0], s->lane_stream[1]};
+    delete s;  // the lanes' path state is released here, with their work finished
+    for (hipStream_t st : streams) (void)hipStreamDestroy(st);
     return pending == hipSuccess ? PT_OK : hip_error(pending, "work pending at server destruction");
 }
 
@@ -419,16 +620,23 @@ pt_status pt_server_submit(pt_server* s, const pt_render_request* r, uint64_t* i
     if (r->max_bounces > PT_MAX_BOUNCES_LIMIT) return set_error(PT_ERR_INVALID, "max bounces above %u", PT_MAX_BOUNCES_LIMIT);
     if ((uint64_t)r->samples * 64u > (1u << 24)) return set_error(PT_ERR_INVALID, "too many samples per request");
     std::lock_guard<std::mutex> lk(s->mu);
-    const uint64_t need = s->job_bytes(*r) + r->scene->footprint;
-    if (need > s->cfg.hbm_budget_bytes) {
+    if (s->stop) return set_error(PT_ERR_STATE, "server is shutting down");
+    const uint64_t buf = s->job_bytes(*r);
+    const uint64_t need = buf + r->scene->footprint;
+    if (need > s->cfg.hbm_budget_bytes || JobArena::round_up(buf) > s->arena.bytes) {
         s->stats.jobs_rejected += 1;
-        return set_error(PT_ERR_MEMORY, "request needs %llu bytes, over the %llu-byte budget",
-                         (unsigned long long)need, (unsigned long long)s->cfg.hbm_budget_bytes);
+        if (need > s->cfg.hbm_budget_bytes)
+            return set_error(PT_ERR_MEMORY, "request needs %llu bytes, over the %llu-byte budget",
+                             (unsigned long long)need, (unsigned long long)s->cfg.hbm_budget_bytes);
+        return set_error(PT_ERR_MEMORY, "request's %llu-byte buffer exceeds the %llu-byte job arena",
+                         (unsigned long long)buf, (unsigned long long)s->arena.bytes);
     }
     ServerJob j;
     j.id = s->next_id++;
     j.req = *r;
     j.submit_s = now_s();
+    j.holds_scene = true;
+    r->scene->server_jobs.fetch_add(1);
     *id = j.id;
     s->jobs.emplace(j.id, j);
     s->queued.push_back(j.id);
@@ -439,15 +647,20 @@ pt_status pt_server_submit(pt_server* s, const pt_render_request* r, uint64_t* i
 pt_status pt_server_wait(pt_server* s, uint64_t id, uint32_t timeout_ms) {
     if (!s) return set_error(PT_ERR_INVALID, "null server");
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->jobs.find(id);
-    if (it == s->jobs.end()) return set_error(PT_ERR_INVALID, "unknown job %llu", (unsigned long long)id);
+    if (s->jobs.find(id) == s->jobs.end()) return set_error(PT_ERR_INVALID, "unknown job %llu", (unsigned long long)id);
+    // another thread may collect the job meanwhile: a missing record ends the wait
     auto finished = [&] {
-        int st = s->jobs.at(id).state;
+        auto it = s->jobs.find(id);
+        if (it == s->jobs.end()) return true;
+        const int st = it->second.state;
         return st == PT_JOB_DONE || st == PT_JOB_FAILED || st == PT_JOB_CANCELLED;
     };
     if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), finished))
         return set_error(PT_ERR_TIMEOUT, "job %llu not done after %u ms", (unsigned long long)id, timeout_ms);
-    const ServerJob& j = s->jobs.at(id);
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end())
+        return set_error(PT_ERR_INVALID, "job %llu was already collected", (unsigned long long)id);
+    const ServerJob& j = it->second;
     if (j.state == PT_JOB_FAILED) return set_error(PT_ERR_DEVICE, "job %llu failed: %s", (unsigned long long)id, j.error.c_str());
     if (j.state == PT_JOB_CANCELLED) return set_error(PT_ERR_CANCELLED, "job %llu was cancelled", (unsigned long long)id);
     return PT_OK;
@@ -465,13 +678,14 @@ pt_status pt_server_info(pt_server* s, uint64_t id, pt_job_info* info) {
     info->admit_s = j.admit_s;
     info->done_s = j.done_s;
     info->footprint_bytes = j.footprint;
+    info->result_bytes = s->job_bytes(j.req);
     return PT_OK;
 }
 
-pt_status pt_server_result(pt_server* s, uint64_t id, float* rgba) {
+pt_status pt_server_result(pt_server* s, uint64_t id, float* rgba, size_t capacity) {
     if (!s) return set_error(PT_ERR_INVALID, "null argument");
     float4* buf = nullptr;
-    size_t bytes = 0;
+    uint64_t bytes = 0, held = 0;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         auto it = s->jobs.find(id);
@@ -487,16 +701,24 @@ pt_status pt_server_result(pt_server* s, uint64_t id, float* rgba) {
         }
         if (j.state != PT_JOB_DONE) return set_error(PT_ERR_STATE, "job %llu is not finished", (unsigned long long)id);
         if (!rgba) return set_error(PT_ERR_INVALID, "null argument");
+        bytes = s->job_bytes(j.req);
+        if (capacity < bytes)
+            return set_error(PT_ERR_INVALID, "result of job %llu needs %llu bytes, the buffer holds %zu",
+                             (unsigned long long)id, (unsigned long long)bytes, capacity);
+        // the record leaves now; the buffer returns to the arena after the copy
         buf = j.accum;
-        bytes = j.footprint;
+        held = j.footprint;
         j.accum = nullptr;
-        s->bytes_in_use -= j.footprint;
         s->jobs.erase(it);
     }
     hipError_t e = hipSetDevice(s->ctx->device);
     if (e == hipSuccess) e = hipMemcpy(rgba, buf, bytes, hipMemcpyDeviceToHost);
-    (void)hipFree(buf);
-    s->cv_work.notify_all();  // memory freed: more jobs may fit
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->arena.give(buf, held);
+        s->bytes_in_use -= held;
+    }
+    s->cv_work.notify_all();  // memory returned: more jobs may fit
     return e == hipSuccess ? PT_OK : hip_error(e, "job result copy");
 }
 
@@ -509,13 +731,14 @@ pt_status pt_server_cancel(pt_server* s, uint64_t id) {
     if (j.state == PT_JOB_QUEUED) {
         j.state = PT_JOB_CANCELLED;
         j.done_s = now_s();
+        s->release_scene(j);
         s->queued.erase(std::remove(s->queued.begin(), s->queued.end(), id), s->queued.end());
         s->cv_done.notify_all();
         s->cv_work.notify_all();  // the next queued job may now be first in line
         return PT_OK;
     }
     if (j.state != PT_JOB_RUNNING) return set_error(PT_ERR_STATE, "job %llu has finished", (unsigned long long)id);
-    // the worker stops scheduling it and releases it after its round in flight
+    // the worker stops scheduling it and releases it after its rounds in flight
     j.cancel = true;
     s->cv_work.notify_all();
     return PT_OK;
@@ -526,6 +749,7 @@ pt_status pt_server_stats_get(pt_server* s, pt_server_stats* out) {
     std::lock_guard<std::mutex> lk(s->mu);
     *out = s->stats;
     out->bytes_in_use = s->bytes_in_use;
+    if (!s->in_flight.empty()) out->busy_s += now_s() - s->busy_since;
     return PT_OK;
 }
 

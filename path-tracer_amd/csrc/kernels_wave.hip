@@ -29,29 +29,15 @@ namespace ptd {
 
 constexpr uint32_t WAVE_BLOCK = 256;
 // Traversal launches: one wave per block, many more waves than a CU holds at
-// once (TRACE_WAVES_PER_CU; 32 are resident at full occupancy), so the
-// hardware dispatcher keeps handing waves to SIMDs whose waves finish early
-// instead of leaving them idle behind the slowest ray batch of the grid.
+// once (WaveConfig::trace_waves_per_cu; 32 are resident at full occupancy),
+// so the hardware dispatcher keeps handing waves to SIMDs whose waves finish
+// early instead of leaving them idle behind the slowest ray batch of the grid.
+// The shade launches' blocks per CU, their window of queue rounds and the
+// lane-split threshold are WaveConfig fields as well (the autotuner sweeps them).
 #ifndef PT_TRACE_BLOCK
 #define PT_TRACE_BLOCK 64u
 #endif
-#ifndef PT_TRACE_WAVES_PER_CU
-#define PT_TRACE_WAVES_PER_CU 512u
-#endif
-// rounds of WAVE_BLOCK queue entries that a shade block partitions into hits and misses together
-#ifndef PT_SHADE_WINDOW
-#define PT_SHADE_WINDOW 2u
-#endif
-// smallest render (paths) that two lanes share when it fits one or two chunks
-#ifndef PT_LANE_SPLIT_PATHS
-#define PT_LANE_SPLIT_PATHS (1u << 21)
-#endif
-// blocks per CU of the shade launches
-#ifndef PT_SHADE_BLOCKS_PER_CU
-#define PT_SHADE_BLOCKS_PER_CU 32u
-#endif
 constexpr uint32_t TRACE_BLOCK = PT_TRACE_BLOCK;
-constexpr uint32_t SHADE_WINDOW = PT_SHADE_WINDOW;
 constexpr uint32_t MAX_COUNTS = 2 * (PT_MAX_BOUNCES_LIMIT + 2);
 constexpr uint32_t SORT_BUCKETS = 256;
 
@@ -467,7 +453,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_fetch(DevScene s0, const 
 #ifndef PT_SHADE_WAVES_PER_EU
 #define PT_SHADE_WAVES_PER_EU 1
 #endif
-template <bool MULTI, bool INLINE_SHADOW>
+template <bool MULTI, bool INLINE_SHADOW, uint32_t WIN>
 __global__ __launch_bounds__(WAVE_BLOCK, PT_SHADE_WAVES_PER_EU) void k_shade(DevScene s0, const DevScene* __restrict__ scenes,
                                                       const WaveJob* __restrict__ jobs, WaveJob j0, PathState ps,
                                                       uint32_t bounce, const uint32_t* __restrict__ queue,
@@ -488,13 +474,13 @@ __global__ __launch_bounds__(WAVE_BLOCK, PT_SHADE_WAVES_PER_EU) void k_shade(Dev
     BlockQueue nq{nq_buf, &nq_n, &nq_base}, sq{sq_buf, &sq_n, &sq_base};
     bq_init(nq);
     bq_init(sq);
-    // The block takes SHADE_WINDOW rounds of WAVE_BLOCK queue entries at a time
+    // The block takes WIN rounds of WAVE_BLOCK queue entries at a time
     // and reorders them in LDS: paths whose ray hit a surface first, then the
     // missed ones, each in queue order. A wave then runs either the surface
     // work (BSDF setup, E_dielectric, NEE, sampling: thousands of
     // instructions) or the sky lookup, instead of idling its missed lanes
     // through the surface work. Each path's computation is unchanged.
-    constexpr uint32_t WIN = SHADE_WINDOW, NW = WAVE_BLOCK / 64u;
+    constexpr uint32_t NW = WAVE_BLOCK / 64u;
     __shared__ uint32_t win[WIN * WAVE_BLOCK];
     __shared__ uint32_t win_hits[WIN * NW], win_misses[WIN * NW];
     const uint32_t wave = threadIdx.x >> 6;
@@ -836,12 +822,18 @@ void run_chunk(Wavefront& w, const Launch& L, uint32_t max_bounces, bool countin
         }
         {
             EventScope tm(lg, EV_SHADE, stream);
-            if (L.any_alpha)
-                k_shade<MULTI, true><<<L.shade_grid, WAVE_BLOCK, 0, stream>>>(*L.s0, L.scenes, L.jobs, *L.j0, ps, b, cur, cnt, nxt,
-                                                                      w.counts + 2 * (b + 1), shadow, cnt + 1, w.ovf);
-            else
-                k_shade<MULTI, false><<<L.shade_grid, WAVE_BLOCK, 0, stream>>>(*L.s0, L.scenes, L.jobs, *L.j0, ps, b, cur, cnt,
-                                                                       nxt, w.counts + 2 * (b + 1), shadow, cnt + 1, w.ovf);
+#define PT_SHADE_LAUNCH(A, W)                                                                                   \
+    k_shade<MULTI, A, W><<<L.shade_grid, WAVE_BLOCK, 0, stream>>>(*L.s0, L.scenes, L.jobs, *L.j0, ps, b, cur, cnt, nxt, \
+                                                                  w.counts + 2 * (b + 1), shadow, cnt + 1, w.ovf)
+            switch ((L.any_alpha ? 8u : 0u) | w.cfg.shade_window) {
+                case 1: PT_SHADE_LAUNCH(false, 1); break;
+                case 4: PT_SHADE_LAUNCH(false, 4); break;
+                case 8 | 1: PT_SHADE_LAUNCH(true, 1); break;
+                case 8 | 4: PT_SHADE_LAUNCH(true, 4); break;
+                case 8 | 2: PT_SHADE_LAUNCH(true, 2); break;
+                default: PT_SHADE_LAUNCH(false, 2); break;
+            }
+#undef PT_SHADE_LAUNCH
         }
         std::swap(cur, nxt);
     }
@@ -993,16 +985,17 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
     // automatically for a render of three chunks or more, or of at least
     // LANE_SPLIT_PATHS paths in one or two chunks, which is then cut into two
     // equal chunks (a 1080p 8-spp job: 2.14 -> 2.20 G path samples/s).
-    constexpr uint64_t LANE_SPLIT_PATHS = PT_LANE_SPLIT_PATHS;
-    const bool split = total_paths <= 2ull * chunk_cap &&
+    const uint64_t LANE_SPLIT_PATHS = cfg.lane_split_paths;
+    const bool split = !one_lane && total_paths <= 2ull * chunk_cap &&
                        (cfg.streams == 2 || (cfg.streams == 0 && total_paths >= LANE_SPLIT_PATHS));
-    const uint32_t lanes = cfg.streams == 2 || (cfg.streams == 0 && (split || total_paths > 2ull * chunk_cap)) ? 2u : 1u;
+    const uint32_t lanes =
+        !one_lane && (cfg.streams == 2 || (cfg.streams == 0 && (split || total_paths > 2ull * chunk_cap))) ? 2u : 1u;
     if (split) chunk_cap = std::max<uint32_t>(max_unit_paths, (uint32_t)((total_paths + 1) / 2));
     pt_status st = reserve(chunk_cap);
     if (st) return st;
     const uint32_t grid = ctx->cu_count * 8u;
-    const uint32_t trace_grid = ctx->cu_count * PT_TRACE_WAVES_PER_CU / (TRACE_BLOCK / 64u);
-    const uint32_t shade_grid = ctx->cu_count * PT_SHADE_BLOCKS_PER_CU;
+    const uint32_t trace_grid = ctx->cu_count * cfg.trace_waves_per_cu / (TRACE_BLOCK / 64u);
+    const uint32_t shade_grid = ctx->cu_count * cfg.shade_blocks_per_cu;
     const uint32_t lds_min = std::min(cfg.lds_depth, MEGA_LDS_DEPTH);
     const size_t need = (size_t)(max_depth > lds_min ? max_depth - lds_min : 1u) *
                         std::max<size_t>((size_t)shade_grid * WAVE_BLOCK, (size_t)trace_grid * TRACE_BLOCK) * 4u;

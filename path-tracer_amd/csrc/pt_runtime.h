@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 
 #include "kernels.h"
@@ -43,7 +44,12 @@ struct Scene {
     uint32_t alias_count = 0;  // scene alias entries currently on the device
     uint64_t footprint = 0;
     uint64_t version = 0;  // bumped by every update; tracers reset accumulation on change
+    // render-server jobs over this scene that are not yet done, failed or
+    // cancelled; destruction and updates are refused while nonzero
+    std::atomic<uint32_t> server_jobs{0};
 };
+// PT_ERR_STATE when server jobs still use the scene
+pt_status scene_require_idle(const Scene* s, const char* what);
 
 // RAII device allocation for temporaries of one call.
 struct DeviceBuffer {

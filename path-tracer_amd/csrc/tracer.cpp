@@ -321,6 +321,22 @@ pt_status pt_tracer_display_rgba8(pt_tracer* t, uint8_t* rgba) {
     return e == hipSuccess ? PT_OK : hip_error(e, "display");
 }
 
+pt_status pt_tonemap_rgba8(pt_context* c, const float* rgba, uint32_t n, uint8_t* out) {
+    if (!c || !rgba || !out) return set_error(PT_ERR_INVALID, "null argument");
+    if (n == 0) return PT_OK;
+    if (pt_status dst = use_device(c->device)) return dst;
+    DeviceBuffer buf;
+    pt_status st = buf.alloc((size_t)n * (sizeof(float4) + 4));
+    if (st) return st;
+    float4* din = (float4*)buf.ptr;
+    uchar4* dout = (uchar4*)(din + n);
+    hipError_t e = hipMemcpyAsync(din, rgba, (size_t)n * sizeof(float4), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_display(din, n, dout, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? PT_OK : hip_error(e, "tone map");
+}
+
 pt_status pt_tracer_primary_hits(pt_tracer* t, uint32_t sample_count, int32_t* tri, float* tt) {
     if (!t || !tri || !tt) return set_error(PT_ERR_INVALID, "null argument");
     if (sample_count == 0) return set_error(PT_ERR_INVALID, "sample_count is 1-based");

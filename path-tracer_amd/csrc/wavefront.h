@@ -115,6 +115,7 @@ struct Wavefront {
     size_t staging_bytes = 0;
     ptd::WaveConfig cfg;
     hipStream_t stream_override = nullptr;  // run on this stream instead of the context's
+    bool one_lane = false;                  // never split a render over a second lane (the caller pipelines)
     // second chunk lane (cfg.streams == 2): its own path state, overflow
     // stacks and stream; odd chunks run there, overlapping the even ones
     Wavefront* partner = nullptr;
@@ -125,6 +126,10 @@ struct Wavefront {
     // Trace every job completely. profiling: 0 off, 1 per-kernel event
     // timing into `log`, 2 timing + traversal counters (synchronises per chunk).
     pt_status render(const std::vector<HostJob>& jobs, int profiling, EventLog* log, unsigned long long* host_counters);
+    // device bytes held: path state, overflow stacks and tables, with the second lane's
+    size_t device_bytes() const {
+        return arena_bytes + ovf_bytes + tables_bytes + (partner ? partner->device_bytes() : 0);
+    }
     ~Wavefront();
 };
 

@@ -130,7 +130,8 @@ class TraceStats(C.Structure):
 
 class WaveConfig(C.Structure):
     _fields_ = [("lds_stack_depth", C.c_uint32), ("chunk_paths", C.c_uint32), ("sort_key", C.c_uint32),
-                ("traversal", C.c_uint32), ("streams", C.c_uint32)]
+                ("traversal", C.c_uint32), ("streams", C.c_uint32), ("trace_waves_per_cu", C.c_uint32),
+                ("shade_blocks_per_cu", C.c_uint32), ("shade_window", C.c_uint32), ("lane_split_paths", C.c_uint32)]
 
 
 class TuneResult(C.Structure):
@@ -138,7 +139,8 @@ class TuneResult(C.Structure):
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active_jobs", C.c_uint32), ("round_paths", C.c_uint32)]
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active_jobs", C.c_uint32), ("round_paths", C.c_uint32),
+                ("job_arena_bytes", C.c_uint64)]
 
 
 class RenderRequest(C.Structure):
@@ -148,12 +150,13 @@ class RenderRequest(C.Structure):
 
 class JobInfo(C.Structure):
     _fields_ = [("state", C.c_int32), ("rounds", C.c_uint32), ("submit_s", C.c_double), ("admit_s", C.c_double),
-                ("done_s", C.c_double), ("footprint_bytes", C.c_uint64)]
+                ("done_s", C.c_double), ("footprint_bytes", C.c_uint64), ("result_bytes", C.c_uint64)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("rounds", C.c_uint64), ("paths", C.c_uint64), ("jobs_done", C.c_uint64),
-                ("jobs_rejected", C.c_uint64), ("busy_s", C.c_double), ("bytes_in_use", C.c_uint64)]
+                ("jobs_rejected", C.c_uint64), ("busy_s", C.c_double), ("bytes_in_use", C.c_uint64),
+                ("max_jobs_per_round", C.c_uint32), ("max_rounds_in_flight", C.c_uint32), ("arena_bytes", C.c_uint64)]
 
 
 class CameraParams(C.Structure):
@@ -230,12 +233,13 @@ _SIGNATURES = [
     ("pt_context_get_wave_config", _i32, [_p, C.POINTER(WaveConfig)]),
     ("pt_trace_batch", _i32, [C.POINTER(_p), _u32, C.POINTER(_u32)]),
     ("pt_autotune_batch", _i32, [C.POINTER(_p), _u32, C.POINTER(_u32), C.POINTER(TuneResult)]),
+    ("pt_tonemap_rgba8", _i32, [_p, _p, _u32, _p]),
     ("pt_server_create", _i32, [_p, C.POINTER(ServerConfig), C.POINTER(_p)]),
     ("pt_server_destroy", _i32, [_p]),
     ("pt_server_submit", _i32, [_p, C.POINTER(RenderRequest), C.POINTER(C.c_uint64)]),
     ("pt_server_wait", _i32, [_p, C.c_uint64, _u32]),
     ("pt_server_info", _i32, [_p, C.c_uint64, C.POINTER(JobInfo)]),
-    ("pt_server_result", _i32, [_p, C.c_uint64, _p]),
+    ("pt_server_result", _i32, [_p, C.c_uint64, _p, C.c_size_t]),
     ("pt_server_cancel", _i32, [_p, C.c_uint64]),
     ("pt_server_stats_get", _i32, [_p, C.POINTER(ServerStats)]),
     ("pt_assets_create", _i32, [C.POINTER(_p)]),
@@ -731,8 +735,9 @@ class Scene:
         return out
 
     def close(self):
+        """Free the device scene; PtError (PT_ERR_STATE, scene kept) while server jobs use it."""
         if self.handle:
-            lib().pt_scene_destroy(self.handle)
+            _check(lib().pt_scene_destroy(self.handle))
             self.handle = None
 
     def __del__(self):
@@ -836,13 +841,17 @@ class Tracer:
             pass
 
 
-DEFAULT_WAVE_CONFIG = dict(lds_stack_depth=16, chunk_paths=1 << 24, sort_key=SORT_NONE, traversal=0, streams=0)
+DEFAULT_WAVE_CONFIG = dict(lds_stack_depth=16, chunk_paths=1 << 24, sort_key=SORT_NONE, traversal=0, streams=0,
+                           trace_waves_per_cu=512, shade_blocks_per_cu=32, shade_window=2, lane_split_paths=1 << 21)
 
 
 def set_wave_config(ctx: Context, **overrides):
     """Set the context's wavefront tuning; unspecified knobs take the defaults."""
+    unknown = set(overrides) - set(DEFAULT_WAVE_CONFIG)
+    if unknown:
+        raise ValueError(f"unknown wave config fields: {sorted(unknown)}")
     v = {**DEFAULT_WAVE_CONFIG, **overrides}
-    cfg = WaveConfig(v["lds_stack_depth"], v["chunk_paths"], v["sort_key"], v["traversal"], v["streams"])
+    cfg = WaveConfig(*(v[k] for k, _ in WaveConfig._fields_))
     _check(lib().pt_context_set_wave_config(ctx.handle, C.byref(cfg)))
 
 
@@ -850,6 +859,16 @@ def get_wave_config(ctx: Context):
     cfg = WaveConfig()
     _check(lib().pt_context_get_wave_config(ctx.handle, C.byref(cfg)))
     return {k: getattr(cfg, k) for k, _ in WaveConfig._fields_}
+
+
+def tonemap_rgba8(ctx: Context, rgba):
+    """The display pass (ACES + sRGB encode, RGBA8) of float texels (..., 4) on the device."""
+    a = np.ascontiguousarray(rgba, dtype=np.float32)
+    if a.shape[-1] != 4:
+        raise ValueError("texels must have 4 channels")
+    out = np.empty(a.shape, np.uint8)
+    _check(lib().pt_tonemap_rgba8(ctx.handle, _ptr(a), a.size // 4, _ptr(out)))
+    return out
 
 
 def _tracer_array(tracers, samples):
@@ -874,19 +893,28 @@ def autotune_batch(tracers, samples):
 
 
 class Server:
-    """Render server with continuous batching (pt_server_*)."""
+    """Render server with continuous batching (pt_server_*).
 
-    def __init__(self, ctx: Context, hbm_budget_bytes=200 << 30, max_active_jobs=64, round_paths=0):
+    The server keeps each submitted job's Scene referenced until the job is
+    collected (result) or the server is closed, so a scene cannot be garbage
+    collected while a job over it is queued or running."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes=200 << 30, max_active_jobs=64, round_paths=0,
+                 job_arena_bytes=0):
         h = C.c_void_p()
-        cfg = ServerConfig(hbm_budget_bytes, max_active_jobs, round_paths)
+        cfg = ServerConfig(hbm_budget_bytes, max_active_jobs, round_paths, job_arena_bytes)
         _check(lib().pt_server_create(ctx.handle, C.byref(cfg), C.byref(h)))
         self.handle = h
         self.ctx = ctx
+        self._jobs = {}  # job id -> (scene, width, height)
+        self._lock = __import__("threading").Lock()
 
     def submit(self, scene: Scene, width, height, samples, max_bounces):
         req = RenderRequest(scene.handle, width, height, samples, max_bounces)
         jid = C.c_uint64()
-        _check(lib().pt_server_submit(self.handle, C.byref(req), C.byref(jid)))
+        with self._lock:
+            _check(lib().pt_server_submit(self.handle, C.byref(req), C.byref(jid)))
+            self._jobs[jid.value] = (scene, width, height)
         return jid.value
 
     def wait(self, job_id, timeout_ms=60_000):
@@ -897,9 +925,23 @@ class Server:
         _check(lib().pt_server_info(self.handle, job_id, C.byref(i)))
         return {k: getattr(i, k) for k, _ in JobInfo._fields_}
 
-    def result(self, job_id, width, height):
+    def result(self, job_id):
+        """The done job's accumulation (height x width x 4 float32); releases the job."""
+        with self._lock:
+            entry = self._jobs.get(job_id)
+        if entry is None:
+            _check(lib().pt_server_result(self.handle, job_id, None, 0))  # raises: unknown or collected
+        _, width, height = entry
         out = np.empty((height, width, 4), np.float32)
-        _check(lib().pt_server_result(self.handle, job_id, _ptr(out)))
+        try:
+            _check(lib().pt_server_result(self.handle, job_id, _ptr(out), out.nbytes))
+        except PtError as e:
+            if e.code != PT_ERR_STATE:  # the record is gone: failed, cancelled (or collected elsewhere)
+                with self._lock:
+                    self._jobs.pop(job_id, None)
+            raise
+        with self._lock:
+            self._jobs.pop(job_id, None)
         return out
 
     def cancel(self, job_id):
@@ -914,6 +956,7 @@ class Server:
         if self.handle:
             lib().pt_server_destroy(self.handle)
             self.handle = None
+            self._jobs.clear()
 
     def __del__(self):
         try:

@@ -133,7 +133,7 @@ def test_server_result_same_alone_and_in_the_mix(ctx, scenes):
     try:
         a = srv.submit(sc[si], w, h, spp, mb)
         srv.wait(a)
-        alone = srv.result(a, w, h)
+        alone = srv.result(a)
         # the same request among others submitted concurrently from threads
         ids = {}
 
@@ -148,7 +148,7 @@ def test_server_result_same_alone_and_in_the_mix(ctx, scenes):
         for k, jid in ids.items():
             srv.wait(jid)
         infos = {k: srv.info(jid) for k, jid in ids.items()}
-        mixed = {k: srv.result(jid, JOBS[k][1], JOBS[k][2]) for k, jid in ids.items()}
+        mixed = {k: srv.result(jid) for k, jid in ids.items()}
         stats = srv.stats()
     finally:
         srv.close()
@@ -191,7 +191,7 @@ def test_server_rejects_over_budget_and_cancels(ctx, scenes):
         assert e.value.code == ptcore.PT_ERR_STATE
         # collecting a cancelled job reports it and releases the record
         with pytest.raises(ptcore.PtError) as e:
-            srv.result(queued[1], 64, 64)
+            srv.result(queued[1])
         assert e.value.code == ptcore.PT_ERR_CANCELLED
         with pytest.raises(ptcore.PtError) as e:
             srv.info(queued[1])
@@ -210,10 +210,10 @@ def test_server_idle_scratch_does_not_block_admission(ctx, scenes):
     try:
         a = srv.submit(sc[0], 64, 64, 2, 2)
         srv.wait(a, 30_000)
-        first = srv.result(a, 64, 64)
+        first = srv.result(a)
         b = srv.submit(sc[0], 64, 64, 2, 2)
         srv.wait(b, 30_000)
-        second = srv.result(b, 64, 64)
+        second = srv.result(b)
     finally:
         srv.close()
     assert np.array_equal(first, second)
@@ -236,7 +236,7 @@ def test_server_cancels_a_running_job(ctx, scenes):
             srv.wait(big)
         assert e.value.code == ptcore.PT_ERR_CANCELLED
         srv.wait(small)
-        got = srv.result(small, 64, 64)
+        got = srv.result(small)
         stats = srv.stats()
     finally:
         srv.close()
@@ -296,11 +296,11 @@ def test_server_concurrent_submit_and_cancel(ctx, scenes):
         for jid, job, cancel in submitted:
             try:
                 srv.wait(jid, 60_000)
-                done.append((job, srv.result(jid, job[1], job[2])))
+                done.append((job, srv.result(jid)))
             except ptcore.PtError as e:
                 assert cancel and e.code == ptcore.PT_ERR_CANCELLED
                 with pytest.raises(ptcore.PtError):
-                    srv.result(jid, job[1], job[2])  # reports the cancellation and releases the record
+                    srv.result(jid)  # reports the cancellation and releases the record
         stats = srv.stats()
     finally:
         srv.close()
@@ -308,3 +308,170 @@ def test_server_concurrent_submit_and_cancel(ctx, scenes):
     assert len(done) >= len(submitted) // 2
     for job, img in done[:4]:
         assert np.array_equal(img, render_alone(ctx, sc, [job])[0])
+
+
+def test_server_job_matches_oracle(ctx, scenes):
+    """A server job's image equals the CPU oracle's render of the same request, bit for bit."""
+    worlds, sc = scenes
+    si, w, h, spp, mb = JOBS[0]
+    srv = ptcore.Server(ctx, hbm_budget_bytes=1 << 34, max_active_jobs=4, round_paths=1 << 14)
+    try:
+        ids = [srv.submit(sc[si], w, h, spp, mb), srv.submit(sc[1], 64, 48, 3, 4)]
+        for jid in ids:
+            srv.wait(jid)
+        got = srv.result(ids[0])
+    finally:
+        srv.close()
+    ref = oracle.render_f32(worlds[si].desc, w, h, mb, spp)[0]
+    assert np.array_equal(got[..., :3], ref[..., :3])
+
+
+def test_server_shares_rounds_and_overlaps_lanes(ctx, scenes):
+    """Jobs submitted together share rounds (a round holds blocks of several
+    jobs) and two rounds are on the device at once; every image still equals
+    its lone render."""
+    _, sc = scenes
+    jobs = [(2, 64, 64, 16, 6), (1, 72, 40, 3, 3), (0, 33, 17, 5, 2), (1, 128, 96, 2, 6), (2, 96, 64, 4, 6),
+            (0, 96, 80, 4, 8)]
+    srv = ptcore.Server(ctx, hbm_budget_bytes=1 << 34, max_active_jobs=16, round_paths=1 << 16)
+    try:
+        ids = [srv.submit(sc[j[0]], *j[1:]) for j in jobs]
+        for jid in ids:
+            srv.wait(jid)
+        got = [srv.result(jid) for jid in ids]
+        stats = srv.stats()
+    finally:
+        srv.close()
+    print(f"server stats: {stats}", flush=True)
+    assert stats["max_jobs_per_round"] >= 2
+    assert stats["max_rounds_in_flight"] == 2
+    assert stats["rounds"] >= 3 and stats["bytes_in_use"] == 0
+    for img, ref, job in zip(got, render_alone(ctx, sc, jobs), jobs):
+        assert np.array_equal(img, ref), job
+
+
+def test_server_refuses_scene_edits_and_destruction_while_jobs_use_it(ctx):
+    """While a job over a scene is queued or running, destroying or updating
+    the scene is refused (PT_ERR_STATE) and the job's image is unaffected;
+    once the job is done both are allowed again."""
+    world = ptcore.World.generate(ptcore.SCENE_SIZED, 21, 256, 256, 60000)
+    scene = ptcore.Scene(ctx, world.desc)
+    t = ptcore.Tracer(ctx, scene, 256, 256)
+    t.set_max_bounces(6)
+    t.trace_all(16)
+    alone = t.accumulation()
+    t.close()
+    srv = ptcore.Server(ctx, hbm_budget_bytes=1 << 34, max_active_jobs=2, round_paths=1 << 14)
+    try:
+        jid = srv.submit(scene, 256, 256, 16, 6)  # 1M paths in 16K-path rounds: many rounds
+        cam = ptcore.make_camera((0.0, 1.0, 3.0))
+        for call in (lambda: scene.update_camera(cam), lambda: scene.close()):
+            with pytest.raises(ptcore.PtError) as e:
+                call()
+            assert e.value.code == ptcore.PT_ERR_STATE
+        assert scene.handle  # still alive
+        srv.wait(jid, 120_000)
+        got = srv.result(jid)
+        scene.update_camera(cam)  # allowed once the job is done
+    finally:
+        srv.close()
+    scene.close()
+    assert np.array_equal(got, alone)
+
+
+def test_server_keeps_submitted_scenes_alive(ctx):
+    """The Python server holds a job's scene, so dropping the caller's last
+    reference while the job is pending does not free the scene under it."""
+    import gc
+
+    world = ptcore.World.generate(ptcore.SCENE_CORNELL, 1, 64, 64)
+    scene = ptcore.Scene(ctx, world.desc)
+    srv = ptcore.Server(ctx, hbm_budget_bytes=1 << 34, max_active_jobs=1, round_paths=1 << 12)
+    try:
+        jid = srv.submit(scene, 64, 64, 8, 4)
+        del scene
+        gc.collect()
+        srv.wait(jid)
+        got = srv.result(jid)
+    finally:
+        srv.close()
+    ref = oracle.render_f32(world.desc, 64, 64, 4, 8)[0]
+    assert np.array_equal(got[..., :3], ref[..., :3])
+
+
+def test_server_wait_and_collect_from_two_threads(ctx, scenes):
+    """One thread waits on a job while another collects it: the waiter gets
+    the result or 'already collected' (PT_ERR_INVALID), never a crash."""
+    _, sc = scenes
+    srv = ptcore.Server(ctx, hbm_budget_bytes=1 << 34, max_active_jobs=2, round_paths=1 << 12)
+    outcomes = []
+    try:
+        for _ in range(8):
+            jid = srv.submit(sc[0], 64, 64, 2, 2)
+
+            def waiter():
+                try:
+                    srv.wait(jid, 60_000)
+                    outcomes.append("ok")
+                except ptcore.PtError as e:
+                    outcomes.append(e.code)
+
+            th = threading.Thread(target=waiter)
+            th.start()
+            while True:
+                try:
+                    srv.result(jid)
+                    break
+                except ptcore.PtError as e:
+                    assert e.code == ptcore.PT_ERR_STATE
+                    time.sleep(0.0005)
+            th.join()
+        assert srv.stats()["bytes_in_use"] == 0
+    finally:
+        srv.close()
+    assert all(o in ("ok", ptcore.PT_ERR_INVALID) for o in outcomes), outcomes
+
+
+def test_server_result_checks_the_buffer_size(ctx, scenes):
+    import ctypes as C
+
+    _, sc = scenes
+    srv = ptcore.Server(ctx, hbm_budget_bytes=1 << 34, max_active_jobs=1)
+    try:
+        jid = srv.submit(sc[0], 48, 32, 2, 2)
+        srv.wait(jid)
+        assert srv.info(jid)["result_bytes"] == 48 * 32 * 16
+        small = np.empty((32, 47, 4), np.float32)
+        rc = ptcore.lib().pt_server_result(srv.handle, jid, small.ctypes.data_as(C.c_void_p), small.nbytes)
+        assert rc == ptcore.PT_ERR_INVALID
+        assert srv.info(jid)["state"] == ptcore.JOB_DONE  # kept
+        got = srv.result(jid)
+    finally:
+        srv.close()
+    assert np.array_equal(got, render_alone(ctx, sc, [(0, 48, 32, 2, 2)])[0])
+
+
+def test_server_arena_limits_admission(ctx, scenes):
+    """Job buffers come from the arena reserved at creation: a request larger
+    than the arena is rejected, and jobs that do not all fit at once wait for
+    buffers to return and still complete exactly."""
+    _, sc = scenes
+    job = (1, 64, 64, 4, 3)
+    arena = 2 * 64 * 64 * 16  # two jobs' buffers
+    srv = ptcore.Server(ctx, hbm_budget_bytes=1 << 34, max_active_jobs=8, round_paths=1 << 13, job_arena_bytes=arena)
+    try:
+        assert srv.stats()["arena_bytes"] == arena
+        with pytest.raises(ptcore.PtError) as e:
+            srv.submit(sc[0], 128, 128, 1, 1)
+        assert e.value.code == ptcore.PT_ERR_MEMORY
+        ids = [srv.submit(sc[job[0]], *job[1:]) for _ in range(6)]
+        got = []
+        for jid in ids:
+            srv.wait(jid, 60_000)
+            got.append(srv.result(jid))
+        assert srv.stats()["bytes_in_use"] == 0
+    finally:
+        srv.close()
+    ref = render_alone(ctx, sc, [job])[0]
+    for img in got:
+        assert np.array_equal(img, ref)
