@@ -260,6 +260,24 @@ pt_status pt_tracer_read_storage(pt_tracer* tracer, float* rgba);
 /* PathTracer::display + readback: ACES filmic tone map and sRGB encode of the
  * storage texture into RGBA8 (display.wgsl:19-46). */
 pt_status pt_tracer_display_rgba8(pt_tracer* tracer, uint8_t* rgba);
+/* No counterpart in the reference: a tile render for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). It traces `samples` consecutive samples,
+ * from the 1-based index params->sample_count, of the tile of params'
+ * image at params->tile_origin, with the megakernel schedule (the same
+ * kernel, and so the same bits, as pt_tracer_trace_all on a megakernel
+ * tracer with that tile). `accum` is the caller's device buffer of
+ * tile_w * tile_h float4, read (when the first sample is > 1) and written as
+ * the running sum; `storage` (device, may be NULL) receives EXPOSURE * sum /
+ * last sample index. The kernel's traversal overflow stacks use a static
+ * device array of 32 MiB per GPU (held until the process ends); the call
+ * allocates and frees nothing, and rejects a scene whose BVH is too deep for
+ * it (PT_ERR_INVALID). Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other contexts, other threads) race. Asynchronous on the
+ * context's stream (pt_context_synchronize orders it with the caller's use of
+ * the buffers). */
+pt_status pt_render_tile_noalloc(pt_scene* scene, const PtTraceParameters* params, uint32_t tile_w, uint32_t tile_h,
+                                 uint32_t samples, float* accum, float* storage);
 /* The display pass on host data (display.wgsl:19-46): the ACES fit and the
  * sRGB encode of `count` float4 texels (rgba, count * 4 floats) into RGBA8,
  * rounded to nearest as an Rgba8Unorm target stores them (alpha clamped to

@@ -233,6 +233,7 @@ pt_status pt_context_create(int device, pt_context** out) {
     e = hipMalloc(&c->srgb_lut, sizeof(lut));
     if (e == hipSuccess) e = hipMemcpy(c->srgb_lut, lut, sizeof(lut), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&c->counters, 16 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = noalloc_scratch_address(device, &c->noalloc_scratch);
     if (e != hipSuccess) {
         pt_context_destroy(c);
         return hip_error(e, "context buffers");

@@ -10,6 +10,9 @@ namespace ptd {
 // entries of the per-thread traversal stack kept in LDS by the megakernel and
 // the query kernels; deeper entries go to the global overflow area
 constexpr uint32_t MEGA_LDS_DEPTH = 16;
+// bytes of the static device array that pt_render_tile_noalloc's kernels use
+// for traversal overflow stacks
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(32) << 20;
 
 // One tile of one image (path_tracer.rs:45-57 plus the tile size and the
 // number of consecutive samples to trace in this launch).
@@ -52,5 +55,7 @@ hipError_t launch_intersect(const ptd::DevScene& s, const float* rays, uint32_t 
 hipError_t launch_occluded(const ptd::DevScene& s, const float* rays, const float* tmax, const int32_t* skip,
                            uint32_t count, uint8_t* out, uint32_t* ovf, uint32_t grid, hipStream_t stream);
 hipError_t launch_display(const float4* storage, uint32_t n, uchar4* out, hipStream_t stream);
+// Device address of the no-alloc scratch array on `device` (which must be current).
+hipError_t noalloc_scratch_address(int device, uint32_t** out);
 
 }  // namespace ptcore

@@ -8,7 +8,17 @@
 // the global stack-overflow area is sized by the resident grid, not the tile.
 #include "kernels.h"
 
+#include <mutex>
+
 namespace ptd {
+
+// Device memory of pt_render_tile_noalloc, for callers that cannot allocate:
+// the megakernel's traversal overflow stacks live here instead of in a
+// hipMalloc'ed buffer. Each device holds its own copy for the life of the
+// process. Host code never passes `scratch` itself (its host-side name is not
+// a device address): it takes the per-device address from
+// noalloc_scratch_address() below.
+static __device__ __align__(256) uint32_t scratch[NOALLOC_SCRATCH_BYTES / sizeof(uint32_t)];
 
 __device__ __forceinline__ void tile_pixel(uint32_t block, uint32_t tid, uint32_t blocks_x, uint32_t& gx, uint32_t& gy) {
     const uint32_t wave = tid >> 6, lane = tid & 63u;
@@ -168,6 +178,23 @@ __global__ __launch_bounds__(256) void k_display(const float4* __restrict__ stor
 
 // ------------------------------------------------------------------ launchers
 namespace ptcore {
+
+hipError_t noalloc_scratch_address(int device, uint32_t** out) {
+    // once per device, with that device current (the caller made it current)
+    static std::mutex mu;
+    static uint32_t* addr[64] = {};
+    *out = nullptr;
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(ptd::scratch));
+        if (e != hipSuccess) return e;
+        addr[device] = (uint32_t*)p;
+    }
+    *out = addr[device];
+    return hipSuccess;
+}
 
 hipError_t launch_megakernel(const ptd::DevScene& s, const ptd::TileArgs& a, float4* accum, float4* storage,
                              uint32_t* ovf, uint32_t grid, unsigned long long* counters, hipStream_t stream) {

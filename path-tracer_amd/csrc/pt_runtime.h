@@ -28,6 +28,7 @@ struct Context {
     uint32_t* ovf = nullptr;              // grid-strided traversal overflow stacks
     size_t ovf_bytes = 0;
     unsigned long long* counters = nullptr;  // device counters of the last profiled launch
+    uint32_t* noalloc_scratch = nullptr;     // device address of the no-alloc scratch array
     ptd::WaveConfig wave_cfg;                // tuning of every wavefront render on this context
     struct Wavefront* batch_wave = nullptr;  // path state of pt_trace_batch / autotune
     // make room for `depth` stack entries for `grid_threads` threads

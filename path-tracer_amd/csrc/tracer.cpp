@@ -321,6 +321,48 @@ pt_status pt_tracer_display_rgba8(pt_tracer* t, uint8_t* rgba) {
     return e == hipSuccess ? PT_OK : hip_error(e, "display");
 }
 
+pt_status pt_render_tile_noalloc(pt_scene* s, const PtTraceParameters* p, uint32_t tile_w, uint32_t tile_h,
+                                 uint32_t samples, float* accum, float* storage) {
+    if (!s || !p || !accum) return set_error(PT_ERR_INVALID, "pt_render_tile_noalloc: null argument");
+    pt_status st = validate_tile(tile_w, tile_h);
+    if (st) return st;
+    const uint32_t iw = p->image_size[0], ih = p->image_size[1];
+    if (iw == 0 || ih == 0) return set_error(PT_ERR_INVALID, "image size must be nonzero");
+    if (p->tile_origin[0] >= iw || p->tile_origin[1] >= ih)
+        return set_error(PT_ERR_INVALID, "tile origin must be within the image");
+    if (p->max_bounces > PT_MAX_BOUNCES_LIMIT)
+        return set_error(PT_ERR_INVALID, "max bounces %u above %u", p->max_bounces, PT_MAX_BOUNCES_LIMIT);
+    if (p->sample_count == 0) return set_error(PT_ERR_INVALID, "sample_count is the 1-based index of the first sample");
+    if ((uint64_t)p->sample_count + samples > 0xffffffffull) return set_error(PT_ERR_INVALID, "sample index overflow");
+    if (samples == 0) return PT_OK;
+    Context* c = s->ctx;
+    // the megakernel's overflow stacks: depth beyond the LDS part x grid threads,
+    // held to the static scratch array by the grid size
+    const size_t levels = s->bvh_depth + 1u > ptd::MEGA_LDS_DEPTH ? s->bvh_depth + 1u - ptd::MEGA_LDS_DEPTH : 1u;
+    const size_t per_block = levels * 256u * sizeof(uint32_t);
+    const size_t max_grid = ptd::NOALLOC_SCRATCH_BYTES / per_block;
+    if (max_grid == 0)
+        return set_error(PT_ERR_INVALID, "BVH depth %u needs %zu bytes of traversal stack per block, over the %zu-byte scratch",
+                         s->bvh_depth, per_block, ptd::NOALLOC_SCRATCH_BYTES);
+    if (!c->noalloc_scratch) return set_error(PT_ERR_STATE, "no scratch address on this context's device");
+    if (pt_status dst = use_device(c->device)) return dst;
+    const uint32_t blocks = ((tile_w + 15u) / 16u) * ((tile_h + 15u) / 16u);
+    const uint32_t grid = (uint32_t)std::min<size_t>(std::min<uint32_t>(blocks, c->cu_count * 8u), max_grid);
+    ptd::TileArgs a;
+    a.image_w = iw;
+    a.image_h = ih;
+    a.origin_x = p->tile_origin[0];
+    a.origin_y = p->tile_origin[1];
+    a.tile_w = tile_w;
+    a.tile_h = tile_h;
+    a.max_bounces = p->max_bounces;
+    a.first_sample = p->sample_count;
+    a.samples = samples;
+    const hipError_t e = launch_megakernel(s->view, a, (float4*)accum, (float4*)storage, c->noalloc_scratch, grid,
+                                           nullptr, c->stream);
+    return e == hipSuccess ? PT_OK : hip_error(e, "no-alloc tile render");
+}
+
 pt_status pt_tonemap_rgba8(pt_context* c, const float* rgba, uint32_t n, uint8_t* out) {
     if (!c || !rgba || !out) return set_error(PT_ERR_INVALID, "null argument");
     if (n == 0) return PT_OK;

@@ -234,6 +234,7 @@ _SIGNATURES = [
     ("pt_trace_batch", _i32, [C.POINTER(_p), _u32, C.POINTER(_u32)]),
     ("pt_autotune_batch", _i32, [C.POINTER(_p), _u32, C.POINTER(_u32), C.POINTER(TuneResult)]),
     ("pt_tonemap_rgba8", _i32, [_p, _p, _u32, _p]),
+    ("pt_render_tile_noalloc", _i32, [_p, C.POINTER(PtTraceParameters), _u32, _u32, _u32, _p, _p]),
     ("pt_server_create", _i32, [_p, C.POINTER(ServerConfig), C.POINTER(_p)]),
     ("pt_server_destroy", _i32, [_p]),
     ("pt_server_submit", _i32, [_p, C.POINTER(RenderRequest), C.POINTER(C.c_uint64)]),
