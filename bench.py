@@ -1,24 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark of the path-tracing core on one MI355X.
 
-Workload (default): the ~262k-triangle procedural field scene (512x512-class
-noise heightfield sized to 261,920 triangles with 64 icospheres, 256 emissive
+Headline workload: the ~262k-triangle procedural field scene (a noise
+heightfield sized to 261,920 triangles with 64 icospheres, 256 emissive
 triangles), 1920x1080, 8 samples per pixel, max bounces 6. One step = one
-render job = every sample of the frame, traced from sample 1 (the scene,
-its BVH and light tables are resident in HBM before timing starts).
+render job = every sample of the frame, traced from sample 1 (the scene, its
+BVH and light tables are resident in HBM before timing starts).
 
-Prints one JSON line: path samples/s (value), rays/s, p50/p99 job latency,
-the roofline of the traversal (extend) kernel measured live with HIP events
-on its stream, and the CPU oracle's throughput on the host cores.
+The default run prints one JSON line with the headline (path samples/s,
+rays/s, the traversal kernel's roofline from live HIP events, the CPU
+oracle's throughput on the host cores) and, beside it:
+  large     the 4.2M-triangle textured scene at 3840x2160, 64 spp, depth 8:
+            path samples/s and the traversal roofline (two lanes and one lane)
+  batch     the 64-scene mixed batch: batched wavefront vs the plain
+            megakernel and the wavefront, one scene after another
+  serve     Poisson arrivals over the serving pool at 5, 50 and 200 jobs/s:
+            p50/p99 submit-to-result latency and sustained path samples/s
+  copy_gbs  an event-timed 1 GiB device-to-device copy (the measured HBM ceiling)
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload field|cornell|large|batch|serve|interactive]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--headline-only]
+                    [--workload all|field|cornell|large|batch|serve|interactive|copy]
                     [--schedule wavefront|mega] [--no-cpu-baseline]
 """
 import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -27,15 +35,20 @@ sys.path.insert(0, os.path.join(ROOT, "path-tracer_amd"))
 
 HBM_PEAK_GBS = 8000.0
 PAIR_BYTES, TRI_BYTES = 64, 36  # a child pair of 32-B nodes; one 36-B triangle
+METRIC = "path samples/s @1080p depth 6; rays/s; p50/p99 render-job latency"
 
 WORKLOADS = {
-    # name: (scene kind, width, height, spp, max_bounces, description)
-    "field": ("SCENE_FIELD", 1920, 1080, 8, 6,
+    # name: (scene kind, seed, width, height, spp, max_bounces, description)
+    "field": ("SCENE_FIELD", 2, 1920, 1080, 8, 6,
               "field-262k: 261,920-triangle noise heightfield + 64 icospheres, 256 emissive tris, 1920x1080, 8 spp, depth 6"),
-    "cornell": ("SCENE_CORNELL", 256, 256, 16, 8, "cornell-36: Cornell box, 256x256, 16 spp, depth 8"),
-    "large": ("SCENE_LARGE", 3840, 2160, 64, 8,
+    "cornell": ("SCENE_CORNELL", 1, 256, 256, 16, 8, "cornell-36: Cornell box, 256x256, 16 spp, depth 8"),
+    "large": ("SCENE_LARGE", 3, 3840, 2160, 64, 8,
               "large-4.2M: 4,195,522-triangle heightfield + 4096 icospheres, 8 Perlin textures, 3840x2160, 64 spp, depth 8"),
 }
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def percentile(xs, q):
@@ -45,6 +58,17 @@ def percentile(xs, q):
     k = (len(xs) - 1) * q / 100.0
     lo, hi = math.floor(k), math.ceil(k)
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(world, w, h, bounces, samples, repeats, seconds_per_run=4.0):
@@ -69,20 +93,21 @@ def cpu_baseline(world, w, h, bounces, samples, repeats, seconds_per_run=4.0):
         t0 = time.perf_counter()
         _, st = oracle.render_rows_f32(world.desc, w, h, bounces, samples, row0, row0 + rows, threads=threads)
         runs.append((time.perf_counter() - t0, st))
-        print(f"[bench] CPU baseline run {k + 1}: {runs[-1][0]:.2f} s", file=sys.stderr, flush=True)
+        log(f"CPU baseline run {k + 1}: {runs[-1][0]:.2f} s")
     runs.sort(key=lambda r: r[0])
     dt, st = runs[len(runs) // 2]
     total = sum(r[0] for r in runs)
     what = (f"the benchmark job ({w}x{h}" if rows == h else
             f"a central band of the benchmark job's frame (rows {row0}-{row0 + rows - 1} of {w}x{h}")
     return {"value": w * rows * samples / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"{what}, {samples} spp, depth {bounces}) run {len(runs)} times, "
                       f"median {dt:.2f} s of {total:.1f} s total; rays/s "
                       f"{((st['closest_rays'] + st['shadow_rays']) / dt):.4g}"}
 
 
 def load_traffic(name):
-    """HBM bytes per extend launch from the committed rocprofv3 --pmc summary, if present."""
+    """HBM bytes per traversal launch from the committed rocprofv3 --pmc summary, if present."""
     p = os.path.join(ROOT, "profiles", f"extend_traffic_{name}.json")
     if not os.path.exists(p):
         return None
@@ -90,48 +115,162 @@ def load_traffic(name):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
-def build_worlds(specs):
-    """Prepare many procedural worlds in parallel host threads (the C++ build releases the GIL)."""
-    import ptcore
-    from concurrent.futures import ThreadPoolExecutor
-
-    def one(spec):
-        kind, seed, w, h, tris = spec
-        return ptcore.World.generate(kind, seed, w, h, tris)
-
-    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 4)) as ex:
-        return list(ex.map(one, specs))
+def wave_knobs(args):
+    """The wavefront configuration fields set on the command line."""
+    return {k: v for k, v in (("traversal", args.traversal), ("sort_key", args.sort), ("lds_stack_depth", args.lds),
+                              ("streams", args.streams), ("chunk_paths", args.chunk))
+            if v is not None}
 
 
-def mixed_jobs(seed, count):
-    """The mixed-scene batch: triangle counts log-uniform over 1k-500k,
-    resolutions uniform over {256, 512, 768, 1024}^2, spp uniform over {4, 16, 64}, depth 6."""
-    import random
-
-    rng = random.Random(seed)
-    jobs = []
-    for i in range(count):
-        tris = int(math.exp(rng.uniform(math.log(1000), math.log(500000))))
-        res = rng.choice([256, 512, 768, 1024])
-        spp = rng.choice([4, 16, 64])
-        jobs.append(dict(tris=tris, width=res, height=res, spp=spp, bounces=6, seed=1000 + i))
-    return jobs
-
-
-def bench_batch(args):
-    """Mixed-scene batch: every job traced by the plain single-scene megakernel one after
-    another, then all jobs as one autotuned mixed wavefront batch."""
+# ---------------------------------------------------------------- one render job, timed
+def measure_job(ctx, name, steps, warmup, schedule_name="wavefront"):
+    """Time `steps` render jobs of workload `name` (after `warmup`), with live
+    HIP events around every traversal launch. Returns the measurement dict."""
     import ptcore
 
-    jobs = mixed_jobs(4, args.batch_scenes)
+    kind_name, seed, W, H, SPP, BOUNCES, desc = WORKLOADS[name]
     t0 = time.perf_counter()
-    worlds = build_worlds([(ptcore.SCENE_SIZED, j["seed"], j["width"], j["height"], j["tris"]) for j in jobs])
+    world = ptcore.World.generate(getattr(ptcore, kind_name), seed, W, H)
     prep_s = time.perf_counter() - t0
-    ctx = ptcore.Context(0)
+    t0 = time.perf_counter()
+    scene = ptcore.Scene(ctx, world.desc)
+    ctx.synchronize()
+    upload_s = time.perf_counter() - t0
+    schedule = ptcore.SCHEDULE_WAVEFRONT if schedule_name == "wavefront" else ptcore.SCHEDULE_MEGAKERNEL
+    tracer = ptcore.Tracer(ctx, scene, W, H, schedule)
+    tracer.set_max_bounces(BOUNCES)
+    log(f"{desc}; {world.desc.triangle_count} tris, host prep {prep_s:.2f}s, upload {upload_s:.2f}s")
+
+    def step():
+        tracer.set_sample_count(0)
+        tracer.trace_all(SPP)
+
+    # counting pass: traversal work per step (rendering is deterministic, so
+    # every step issues exactly these rays and tests)
+    tracer.set_profiling(2)
+    step()
+    counts = tracer.stats()
+    log(f"{name} counting pass: {counts}")
+    tracer.set_profiling(0)
+    for _ in range(warmup):
+        step()
+    ctx.synchronize()
+
+    # live HIP events around every traversal launch (the roofline kernel), no
+    # extra synchronisation; other kernels carry no markers in the timed steps
+    tracer.set_profiling(3)
+    lat = []
+    ctx.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        step()
+        ctx.synchronize()
+        lat.append((time.perf_counter() - s0) * 1e3)
+    total_s = time.perf_counter() - t_start
+    ev = tracer.stats()
+    # kernel-class breakdown (reported, not timed): a few more steps with every kernel's events
+    extra_steps = min(steps, 3)
+    tracer.set_profiling(1)
+    for _ in range(extra_steps):
+        step()
+    ctx.synchronize()
+    ev_all = tracer.stats()
+    tracer.set_profiling(0)
+    # the same steps on one lane: nothing overlaps a traversal launch there
+    ev_one = None
+    if schedule == ptcore.SCHEDULE_WAVEFRONT:
+        cfg0 = ptcore.get_wave_config(ctx)
+        ptcore.set_wave_config(ctx, **{**cfg0, "streams": 1})
+        tracer.set_profiling(3)
+        for _ in range(extra_steps):
+            step()
+        ctx.synchronize()
+        ev_one = tracer.stats()
+        tracer.set_profiling(0)
+        ptcore.set_wave_config(ctx, **cfg0)
+
+    samples_per_step = W * H * SPP
+    rays_per_step = counts["closest_rays"] + counts["shadow_rays"]
+    out = {
+        "workload": desc, "width": W, "height": H, "spp": SPP, "max_bounces": BOUNCES,
+        "triangles": int(world.desc.triangle_count), "schedule": schedule_name,
+        "value": samples_per_step * steps / total_s, "ms_per_step": total_s * 1e3 / steps,
+        "rays_per_s": rays_per_step * steps / total_s,
+        "step_p50_ms": percentile(lat, 50), "step_p99_ms": percentile(lat, 99),
+        "host_prep_s": prep_s, "upload_s": upload_s,
+        "traversal_counts_per_step": {k: counts[k] for k in ("closest_rays", "shadow_rays", "pair_tests",
+                                                             "tri_tests", "shadow_pair_tests", "shadow_tri_tests")},
+    }
+    if schedule == ptcore.SCHEDULE_WAVEFRONT:
+        # the traversal kernel walks the closest-hit rays of a bounce and the
+        # shadow rays of the previous one; its bytes are both traversals'
+        bytes_per_step = (PAIR_BYTES * (counts["pair_tests"] + counts["shadow_pair_tests"]) +
+                          TRI_BYTES * (counts["tri_tests"] + counts["shadow_tri_tests"]))
+        launches = max(ev["extend_launches"], 1)
+        launches_per_step = launches / steps
+        bytes_per_launch = bytes_per_step / launches_per_step
+        # the union of the launches' event intervals: the time a traversal
+        # kernel was running, at most the step time even where lanes overlap
+        busy_s_per_step = ev["extend_busy_ms"] / 1e3 / steps
+        busy_per_launch_s = busy_s_per_step / launches_per_step
+        achieved = bytes_per_step / busy_s_per_step / 1e9
+        traffic = load_traffic(name)
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "traffic_frac": (traffic / busy_per_launch_s / 1e9 / HBM_PEAK_GBS) if traffic else None,
+                "kernel": "k_trace (closest-hit rays of bounce b + shadow rays of bounce b-1)",
+                "avg_launch_ms": busy_per_launch_s * 1e3, "launches_per_step": launches_per_step,
+                "algorithmic_bytes_per_launch": bytes_per_launch,
+                "busy_ms_per_step": busy_s_per_step * 1e3,
+                "method": "algorithmic bytes (64 per child-pair test + 36 per triangle test, counted on the device) "
+                          "over the union of the traversal launches' HIP-event intervals in the timed steps",
+                "overlapped_events": {
+                    "avg_launch_ms": ev["extend_ms"] / launches,
+                    "frac": bytes_per_launch / (ev["extend_ms"] / 1e3 / launches) / 1e9 / HBM_PEAK_GBS,
+                    "note": "per-launch event durations; with two lanes they include the other lane's kernels"}}
+        one_launches = max(ev_one["extend_launches"], 1)
+        one_per_step = one_launches / extra_steps
+        one_s = ev_one["extend_ms"] / 1e3 / one_launches
+        roof["one_lane"] = {"achieved": bytes_per_step / one_per_step / one_s / 1e9,
+                            "frac": bytes_per_step / one_per_step / one_s / 1e9 / HBM_PEAK_GBS,
+                            "avg_launch_ms": one_s * 1e3, "launches_per_step": one_per_step,
+                            "traffic_frac": (traffic / one_s / 1e9 / HBM_PEAK_GBS) if traffic else None,
+                            "note": "the same job on one lane in separate steps: no launch overlaps another kernel"}
+        out["roofline"] = roof
+        out["kernel_ms_per_step"] = {"extend_busy": busy_s_per_step * 1e3,
+                                     "extend_events": ev_all["extend_ms"] / extra_steps,
+                                     "shade_events": ev_all["shade_ms"] / extra_steps,
+                                     "total": ev_all["total_ms"] / extra_steps,
+                                     "note": "extend_busy from the timed steps; the rest from separate steps with "
+                                             "every kernel timed (two lanes: event sums may exceed the step)"}
+    else:
+        bytes_per_step = PAIR_BYTES * counts["pair_tests"] + TRI_BYTES * counts["tri_tests"]
+        mk_s = ev["total_ms"] / 1e3 / steps
+        out["kernel_ms_per_step"] = {"megakernel": mk_s * 1e3}
+        out["roofline"] = {"bound": "hbm", "achieved": bytes_per_step / mk_s / 1e9, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "traffic": None, "frac": bytes_per_step / mk_s / 1e9 / HBM_PEAK_GBS,
+                           "kernel": "k_megakernel (all traversal bytes over the whole kernel)"}
+    tracer.close()
+    scene.close()
+    return out, world
+
+
+# ---------------------------------------------------------------- the mixed batch
+def bench_batch(ctx, steps, warmup, count=64, autotune=True):
+    """Mixed-scene batch: every job traced by the plain single-scene megakernel one after
+    another, by the wavefront one after another, then all jobs as one autotuned batch."""
+    import ptcore
+    from ptcore import workloads
+
+    jobs = workloads.mixed_jobs(4, count)
+    t0 = time.perf_counter()
+    worlds = workloads.mixed_worlds(jobs)
+    prep_s = time.perf_counter() - t0
     scenes = [ptcore.Scene(ctx, w.desc) for w in worlds]
     samples = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
-    print(f"[bench] batch of {len(jobs)} scenes, {sum(w.desc.triangle_count for w in worlds)} triangles, "
-          f"{samples / 1e6:.1f}M path samples, host prep {prep_s:.1f}s", file=sys.stderr, flush=True)
+    log(f"batch of {len(jobs)} scenes, {sum(w.desc.triangle_count for w in worlds)} triangles, "
+        f"{samples / 1e6:.1f}M path samples, host prep {prep_s:.1f}s")
 
     def tracers(schedule):
         ts = []
@@ -141,93 +280,101 @@ def bench_batch(args):
             ts.append(t)
         return ts
 
-    # plain method: megakernel, one scene at a time
+    def sequential(ts):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for t, j in zip(ts, jobs):
+            t.set_sample_count(0)
+            t.trace_all(j["spp"])
+        ctx.synchronize()
+        return time.perf_counter() - t0
+
     mk = tracers(ptcore.SCHEDULE_MEGAKERNEL)
-    for t, j in zip(mk[:2], jobs[:2]):  # warm the code objects
+    for t in mk[:2]:  # warm the code objects
         t.trace_all(1)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for t, j in zip(mk, jobs):
-        t.set_sample_count(0)
-        t.trace_all(j["spp"])
-    ctx.synchronize()
-    mega_s = time.perf_counter() - t0
+    mega_s = sequential(mk)
     for t in mk:
         t.close()
-    # wavefront, one scene at a time (no batching)
     wf = tracers(ptcore.SCHEDULE_WAVEFRONT)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for t, j in zip(wf, jobs):
-        t.set_sample_count(0)
-        t.trace_all(j["spp"])
-    ctx.synchronize()
-    single_wf_s = time.perf_counter() - t0
-    # batched: autotune, then timed steps of the whole batch
+    sequential(wf)  # warm the path state
+    single_wf_s = sequential(wf)
     spp = [j["spp"] for j in jobs]
-    tune = ptcore.autotune_batch(wf, spp) if not args.no_autotune else None
-    print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
-    times = []
-    for k in range(args.warmup + args.steps):
-        for t in wf:
-            t.set_sample_count(0)
-        ctx.synchronize()
-        s0 = time.perf_counter()
-        ptcore.trace_batch(wf, spp)
-        ctx.synchronize()
-        if k >= args.warmup:
-            times.append(time.perf_counter() - s0)
-    batch_s = sum(times) / len(times)
+    cfg0 = ptcore.get_wave_config(ctx)
+
+    def batched():
+        times = []
+        for k in range(warmup + steps):
+            for t in wf:
+                t.set_sample_count(0)
+            ctx.synchronize()
+            s0 = time.perf_counter()
+            ptcore.trace_batch(wf, spp)
+            ctx.synchronize()
+            if k >= warmup:
+                times.append(time.perf_counter() - s0)
+        return sum(times) / len(times)
+
+    default_s = batched()
+    tune = None
+    batch_s = default_s
+    if autotune:
+        tune = ptcore.autotune_batch(wf, spp)
+        log(f"autotune: {tune}")
+        batch_s = batched()
+    tuned_cfg = ptcore.get_wave_config(ctx)
+    ptcore.set_wave_config(ctx, **cfg0)
     for t in wf:
         t.close()
-    value = samples / batch_s
+    for s in scenes:
+        s.close()
     return {
-        "metric": "path samples/s @1080p depth 6; rays/s; p50/p99 render-job latency",
-        "value": value, "unit": "path samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": batch_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (seeded procedural scenes)",
-        "config": {"workload": f"mixed batch: {len(jobs)} scenes, triangles log-uniform 1k-500k, resolutions "
-                               f"{{256..1024}}^2, spp {{4,16,64}}, depth 6",
-                   "path_samples_per_step": samples, "wave_config": ptcore.get_wave_config(ctx), "autotune": tune},
+        "workload": f"mixed batch: {len(jobs)} scenes, triangles log-uniform 1k-500k, resolutions "
+                    f"{{256..1024}}^2, spp {{4,16,64}}, depth 6",
+        "path_samples_per_step": samples, "steps": steps,
+        "value": samples / batch_s, "ms_per_step": batch_s * 1e3,
+        "default_config": {"value": samples / default_s, "ms_per_step": default_s * 1e3, "wave_config": cfg0},
+        "tuned_config": tuned_cfg, "autotune": tune,
         "megakernel_sequential": {"value": samples / mega_s, "seconds": mega_s},
         "wavefront_sequential": {"value": samples / single_wf_s, "seconds": single_wf_s},
-        "batched_vs_megakernel": (samples / batch_s) / (samples / mega_s),
+        "batched_vs_megakernel": mega_s / batch_s,
+        "batched_vs_wavefront_sequential": single_wf_s / batch_s,
     }
 
 
-def bench_serve(args):
+# ---------------------------------------------------------------- serving
+def bench_serve(ctx, rates, jobs_per_rate):
     """Serving: Poisson arrivals of render requests over resident scenes (one
     4.2M-triangle scene and two 262k-triangle scenes); each request draws its
     resolution from {256..1024}^2 and spp from {4, 16, 64}, depth 6. For each
-    arrival rate: p50/p99 submit-to-result latency and sustained samples/s."""
-    import random
+    arrival rate: p50/p99 submit-to-result latency and sustained samples/s.
+    The same requests of the highest rate traced as one batch give the
+    ceiling the server is measured against."""
     import threading
 
     import ptcore
+    from ptcore import workloads
 
     t0 = time.perf_counter()
-    worlds = build_worlds([(ptcore.SCENE_LARGE, 3, 1024, 1024, 0), (ptcore.SCENE_FIELD, 2, 1024, 1024, 0),
-                           (ptcore.SCENE_FIELD, 12, 1024, 1024, 0)])
-    print(f"[bench] serving pool prepared in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-    ctx = ptcore.Context(0)
+    worlds = workloads.serve_worlds()
+    log(f"serving pool prepared in {time.perf_counter() - t0:.1f}s")
     scenes = [ptcore.Scene(ctx, w.desc) for w in worlds]
     results = []
-    for lam in args.rates:
+    def count_for(lam):
+        return jobs_per_rate or (500 if 500 / lam <= 10.0 else 200)
+
+    for lam in rates:
         srv = ptcore.Server(ctx, hbm_budget_bytes=200 << 30, max_active_jobs=64)
-        rng = random.Random(int(lam * 1000) + 5)
-        reqs = [(rng.randrange(len(scenes)), rng.choice([256, 512, 768, 1024]), rng.choice([4, 16, 64]))
-                for _ in range(args.jobs_per_rate)]
-        gaps = [rng.expovariate(lam) for _ in reqs]
+        reqs = workloads.serve_requests(lam, count_for(lam))
         ids = []
 
         def producer():
             t_next = time.perf_counter()
-            for (si, res, spp), gap in zip(reqs, gaps):
+            for si, res, spp, gap in reqs:
                 t_next += gap
                 d = t_next - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
-                ids.append((srv.submit(scenes[si], res, res, spp, 6), res, spp))
+                ids.append((srv.submit(scenes[si], res, res, spp, workloads.SERVE_BOUNCES), res, spp))
 
         th = threading.Thread(target=producer)
         th.start()
@@ -244,21 +391,67 @@ def bench_serve(args):
         st = srv.stats()
         srv.close()
         r = {"rate_jobs_per_s": lam, "jobs": len(ids), "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
-             "samples_per_s": samples / (last - first), "rounds": st["rounds"], "busy_s": st["busy_s"]}
-        print(f"[bench] serve {r}", file=sys.stderr, flush=True)
+             "samples_per_s": samples / (last - first), "busy_samples_per_s": st["paths"] / max(st["busy_s"], 1e-9),
+             "rounds": st["rounds"], "busy_s": st["busy_s"], "max_jobs_per_round": st["max_jobs_per_round"],
+             "max_rounds_in_flight": st["max_rounds_in_flight"]}
+        log(f"serve {r}")
         results.append(r)
-    return {"metric": "p50/p99 render-job latency under Poisson arrivals; path samples/s",
-            "unit": "ms", "n_gpus": 1, "data": "synthetic (seeded procedural scenes)",
-            "config": {"workload": "serving: pool of one 4.2M-tri and two 262k-tri resident scenes; requests "
-                                   "{256..1024}^2, spp {4,16,64}, depth 6; 200 GB HBM budget, 64 slots"},
-            "sweep": results}
+    # ceiling: the highest rate's requests as one batch
+    reqs = workloads.serve_requests(rates[-1], count_for(rates[-1]))
+    tracers = []
+    for si, res, spp, _ in reqs:
+        t = ptcore.Tracer(ctx, scenes[si], res, res)
+        t.set_max_bounces(workloads.SERVE_BOUNCES)
+        tracers.append(t)
+    spp = [r[2] for r in reqs]
+    ptcore.trace_batch(tracers, spp)  # warm
+    for t in tracers:
+        t.set_sample_count(0)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ptcore.trace_batch(tracers, spp)
+    ctx.synchronize()
+    batch_s = time.perf_counter() - t0
+    batch_samples = sum(r[1] * r[1] * r[2] for r in reqs)
+    for t in tracers:
+        t.close()
+    for s in scenes:
+        s.close()
+    top = results[-1]
+    return {"workload": "serving: pool of one 4.2M-tri and two 262k-tri resident scenes; requests "
+                        "{256..1024}^2, spp {4,16,64}, depth 6; 200 GB HBM budget, 64 slots; Poisson arrivals",
+            "sweep": results,
+            "same_requests_as_one_batch": {"value": batch_samples / batch_s, "seconds": batch_s,
+                                           "rate_jobs_per_s": rates[-1]},
+            "sustained_vs_batch": top["samples_per_s"] / (batch_samples / batch_s)}
 
 
-def wave_knobs(args):
-    """The wavefront configuration fields set on the command line."""
-    return {k: v for k, v in (("traversal", args.traversal), ("sort_key", args.sort), ("lds_stack_depth", args.lds),
-                              ("streams", args.streams), ("chunk_paths", args.chunk))
-            if v is not None}
+# ---------------------------------------------------------------- streaming copy
+def copy_bandwidth():
+    """An event-timed 1 GiB device-to-device copy in a child process (torch
+    brings its own HIP runtime, kept out of this process): GB/s counts the
+    bytes read plus the bytes written."""
+    code = r'''
+import json, torch
+n = 1 << 28
+a = torch.ones(n, dtype=torch.float32, device="cuda"); b = torch.empty_like(a)
+for _ in range(3): b.copy_(a)
+torch.cuda.synchronize()
+s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+ms = []
+for _ in range(10):
+    s.record(); b.copy_(a); e.record(); e.synchronize(); ms.append(s.elapsed_time(e))
+ms.sort()
+print(json.dumps({"bytes": 2 * n * 4, "ms_median": ms[len(ms) // 2], "ms_min": ms[0]}))
+'''
+    try:
+        out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
+        r = json.loads(out.stdout.strip().splitlines()[-1])
+        return {"value": r["bytes"] / (r["ms_median"] * 1e-3) / 1e9, "best": r["bytes"] / (r["ms_min"] * 1e-3) / 1e9,
+                "unit": "GB/s", "bytes_per_copy": r["bytes"],
+                "method": "torch copy_ of 1 GiB float32, HIP events, median of 10; read + written bytes"}
+    except Exception as e:  # reported, never fatal to the benchmark
+        return {"value": None, "error": str(e)[:200]}
 
 
 def bench_interactive(args):
@@ -312,7 +505,7 @@ def bench_interactive(args):
                               "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
                               "path_samples_per_s": W * H * frames / total_s,
                               "trace_only_ms_per_frame": trace_only_ms}
-        print(f"[bench] interactive {name}: {per_schedule[name]}", file=sys.stderr, flush=True)
+        log(f"interactive {name}: {per_schedule[name]}")
         tracer.close()
     best = max(per_schedule, key=lambda k: per_schedule[k]["frames_per_s"])
     out = {
@@ -330,17 +523,37 @@ def bench_interactive(args):
     return out
 
 
+def headline_line(m, args, ctx):
+    import ptcore
+
+    return {
+        "metric": METRIC, "value": m["value"], "unit": "path samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": m["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": m["workload"], "width": m["width"], "height": m["height"], "spp": m["spp"],
+                   "max_bounces": m["max_bounces"], "triangles": m["triangles"], "schedule": m["schedule"],
+                   "wave_config": ptcore.get_wave_config(ctx)},
+        "rays_per_s": m["rays_per_s"],
+        # back-to-back latency of one job; job latency under arrivals is serve.sweep[*].p50_ms/p99_ms
+        "p50_ms": m["step_p50_ms"], "p99_ms": m["step_p99_ms"],
+        "roofline": m["roofline"], "kernel_ms_per_step": m["kernel_ms_per_step"],
+        "traversal_counts_per_step": m["traversal_counts_per_step"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 5; interactive: 200 frames)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 2; interactive: 20 frames)")
-    ap.add_argument("--workload", default="field",
-                    choices=sorted(WORKLOADS) + ["batch", "serve", "copy", "interactive"])
+    ap.add_argument("--workload", default="all",
+                    choices=["all"] + sorted(WORKLOADS) + ["batch", "serve", "copy", "interactive"])
+    ap.add_argument("--headline-only", action="store_true", help="with --workload all: only the 1080p job")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--rates", type=float, nargs="+", default=[5, 10, 20, 50, 100, 200])
-    ap.add_argument("--jobs-per-rate", type=int, default=40)
+    ap.add_argument("--rates", type=float, nargs="+", default=[5, 50, 200])
+    ap.add_argument("--jobs-per-rate", type=int, default=0,
+                    help="requests per arrival rate (default: 200, and 500 where the arrivals take at most 10 s)")
     ap.add_argument("--schedule", default="wavefront", choices=["wavefront", "mega"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-repeats", type=int, default=5, help="CPU baseline runs of the whole job (median)")
@@ -357,6 +570,9 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(pixel/tile sharding across GPUs is designed but not built)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "copy":
+        print(json.dumps(copy_bandwidth()), flush=True)
+        return
 
     import ptcore
 
@@ -365,161 +581,41 @@ def main():
         return
     args.steps = 5 if args.steps is None else args.steps
     args.warmup = 2 if args.warmup is None else args.warmup
-    if args.workload in ("batch", "serve"):
-        out = bench_batch(args) if args.workload == "batch" else bench_serve(args)
-        print(json.dumps(out), flush=True)
-        return
-    if args.workload == "copy":
-        # calibration for the HBM traffic counters: a plain device copy of known size
-        import torch
-
-        n = 1 << 28  # 1 GiB of float32
-        a = torch.ones(n, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(4):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_per_dispatch": n * 4, "dispatches": 4}), flush=True)
-        return
-    kind_name, W, H, SPP, BOUNCES, desc = WORKLOADS[args.workload]
-    t0 = time.perf_counter()
-    world = ptcore.World.generate(getattr(ptcore, kind_name), {"field": 2, "cornell": 1, "large": 3}[args.workload], W, H)
-    prep_s = time.perf_counter() - t0
     ctx = ptcore.Context(0)
-    t0 = time.perf_counter()
-    scene = ptcore.Scene(ctx, world.desc)
-    ctx.synchronize()
-    upload_s = time.perf_counter() - t0
-    schedule = ptcore.SCHEDULE_WAVEFRONT if args.schedule == "wavefront" else ptcore.SCHEDULE_MEGAKERNEL
     knobs = wave_knobs(args)
     if knobs:
         ptcore.set_wave_config(ctx, **{**ptcore.get_wave_config(ctx), **knobs})
-    tracer = ptcore.Tracer(ctx, scene, W, H, schedule)
-    tracer.set_max_bounces(BOUNCES)
-    print(f"[bench] {desc}; {world.desc.triangle_count} tris, host prep {prep_s:.2f}s, upload {upload_s:.2f}s",
-          file=sys.stderr, flush=True)
-
-    def step():
-        tracer.set_sample_count(0)
-        tracer.trace_all(SPP)
-
-    # counting pass: traversal work per step (rendering is deterministic, so
-    # every step issues exactly these rays and tests)
-    tracer.set_profiling(2)
-    step()
-    counts = tracer.stats()
-    print(f"[bench] counting pass: {counts}", file=sys.stderr, flush=True)
-
-    tracer.set_profiling(0)
-    for _ in range(args.warmup):
-        step()
-    ctx.synchronize()
-    print("[bench] warmup done", file=sys.stderr, flush=True)
-
-    # live HIP events around every traversal launch (the roofline kernel), no
-    # extra synchronisation; other kernels carry no markers in the timed steps
-    tracer.set_profiling(3)
-    lat = []
-    ctx.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        s0 = time.perf_counter()
-        step()
-        ctx.synchronize()
-        lat.append((time.perf_counter() - s0) * 1e3)
-    total_s = time.perf_counter() - t_start
-    ev = tracer.stats()
-    # kernel-class breakdown (reported, not timed): a few more steps with every kernel's events
-    breakdown_steps = min(args.steps, 3)
-    tracer.set_profiling(1)
-    for _ in range(breakdown_steps):
-        step()
-    ctx.synchronize()
-    ev_all = tracer.stats()
-    tracer.set_profiling(0)
-    # the same steps on one lane: with two lanes every traversal launch shares
-    # the GPU with the other lane's kernels, so its own duration reads longer
-    ev_one = None
-    if schedule == ptcore.SCHEDULE_WAVEFRONT:
-        cfg0 = ptcore.get_wave_config(ctx)
-        ptcore.set_wave_config(ctx, **{**cfg0, "streams": 1})
-        tracer.set_profiling(3)
-        for _ in range(breakdown_steps):
-            step()
-        ctx.synchronize()
-        ev_one = tracer.stats()
-        tracer.set_profiling(0)
-        ptcore.set_wave_config(ctx, **cfg0)
-
-    samples_per_step = W * H * SPP
-    rays_per_step = counts["closest_rays"] + counts["shadow_rays"]
-    value = samples_per_step * args.steps / total_s
-    ms_per_step = total_s * 1e3 / args.steps
-    result = {
-        "metric": "path samples/s @1080p depth 6; rays/s; p50/p99 render-job latency",
-        "value": value,
-        "unit": "path samples/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": desc, "width": W, "height": H, "spp": SPP, "max_bounces": BOUNCES,
-                   "triangles": int(world.desc.triangle_count), "schedule": args.schedule,
-                   "wave_config": ptcore.get_wave_config(ctx)},
-        "rays_per_s": rays_per_step * args.steps / total_s,
-        "p50_ms": percentile(lat, 50),
-        "p99_ms": percentile(lat, 99),
-    }
-    if schedule == ptcore.SCHEDULE_WAVEFRONT:
-        # the traversal kernel walks the closest-hit rays of a bounce and the
-        # shadow rays of the previous one; its bytes are both traversals'
-        extend_launches = max(ev["extend_launches"], 1)
-        bytes_per_step = (PAIR_BYTES * (counts["pair_tests"] + counts["shadow_pair_tests"]) +
-                          TRI_BYTES * (counts["tri_tests"] + counts["shadow_tri_tests"]))
-        launches_per_step = extend_launches / args.steps
-        bytes_per_launch = bytes_per_step / launches_per_step
-        avg_launch_s = ev["extend_ms"] / 1e3 / extend_launches
-        achieved = bytes_per_launch / avg_launch_s / 1e9
-        result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(args.workload),
-                              "kernel": "k_trace (closest-hit rays of bounce b + shadow rays of bounce b-1)",
-                              "avg_launch_ms": avg_launch_s * 1e3, "launches_per_step": launches_per_step,
-                              "algorithmic_bytes_per_launch": bytes_per_launch}
-        one_launches = max(ev_one["extend_launches"], 1)
-        one_bytes = bytes_per_step / (one_launches / breakdown_steps)
-        one_s = ev_one["extend_ms"] / 1e3 / one_launches
-        result["roofline"]["one_lane"] = {
-            "achieved": one_bytes / one_s / 1e9, "frac": one_bytes / one_s / 1e9 / HBM_PEAK_GBS,
-            "avg_launch_ms": one_s * 1e3, "launches_per_step": one_launches / breakdown_steps,
-            "note": "the same job on one lane in separate steps; the figures above come from the timed steps, "
-                    "whose launches overlap the other lane's kernels when the render runs on two lanes"}
-        result["kernel_ms_per_step"] = {"extend": ev["extend_ms"] / args.steps,
-                                        "shade": ev_all["shade_ms"] / breakdown_steps,
-                                        "shadow": ev_all["shadow_ms"] / breakdown_steps,
-                                        "total": ev_all["total_ms"] / breakdown_steps,
-                                        "note": "extend from the timed steps; the rest from separate steps with "
-                                                "every kernel timed"}
-    else:
-        result["kernel_ms_per_step"] = {"megakernel": ev["total_ms"] / args.steps}
-        bytes_per_step = PAIR_BYTES * counts["pair_tests"] + TRI_BYTES * counts["tri_tests"]
-        result["roofline"] = {"bound": "hbm", "achieved": bytes_per_step / (ev["total_ms"] / 1e3 / args.steps) / 1e9,
-                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
-                              "kernel": "k_megakernel (all traversal bytes over the whole kernel)"}
-        result["roofline"]["frac"] = result["roofline"]["achieved"] / HBM_PEAK_GBS
-    result["traversal_counts_per_step"] = {k: counts[k] for k in ("closest_rays", "shadow_rays", "pair_tests",
-                                                                   "tri_tests", "shadow_pair_tests", "shadow_tri_tests")}
+    if args.workload == "batch":
+        out = bench_batch(ctx, args.steps, args.warmup, args.batch_scenes, not args.no_autotune)
+        print(json.dumps({"metric": METRIC, "unit": "path samples/s", "n_gpus": 1, "dtype": "f32", **out}), flush=True)
+        return
+    if args.workload == "serve":
+        out = bench_serve(ctx, args.rates, args.jobs_per_rate)
+        print(json.dumps({"metric": "p50/p99 render-job latency under Poisson arrivals; path samples/s",
+                          "unit": "ms", "n_gpus": 1, **out}), flush=True)
+        return
+    name = "field" if args.workload == "all" else args.workload
+    m, world = measure_job(ctx, name, args.steps, args.warmup, args.schedule)
+    result = headline_line(m, args, ctx)
+    log(f"headline: {m['value'] / 1e9:.3f} G path samples/s, traversal frac {m['roofline']['frac']:.3f}")
+    if args.workload == "all" and not args.headline_only:
+        lg, _ = measure_job(ctx, "large", 2, 1, args.schedule)
+        result["large"] = {k: lg[k] for k in ("workload", "value", "ms_per_step", "rays_per_s", "roofline",
+                                              "traversal_counts_per_step")}
+        log(f"large: {lg['value'] / 1e9:.3f} G path samples/s, traversal frac {lg['roofline']['frac']:.3f}")
+        result["batch"] = bench_batch(ctx, 2, 1, args.batch_scenes, not args.no_autotune)
+        log(f"batch: {result['batch']['value'] / 1e9:.3f} G path samples/s")
+        result["serve"] = bench_serve(ctx, args.rates, args.jobs_per_rate)
+    if args.workload == "all":
+        result["copy_gbs"] = copy_bandwidth()
+        log(f"copy: {result['copy_gbs']}")
+    result["cpu_model"] = cpu_model()
     if not args.no_cpu_baseline:
-        print("[bench] CPU baseline (oracle on host cores)...", file=sys.stderr, flush=True)
+        log("CPU baseline (oracle on host cores)...")
+        _, _, W, H, SPP, BOUNCES, _ = WORKLOADS[name]
         result["cpu_baseline"] = cpu_baseline(world, W, H, BOUNCES, SPP, args.cpu_repeats)
     else:
         result["cpu_baseline"] = None
-    tracer.close()
-    scene.close()
     ctx.close()
     print(json.dumps(result), flush=True)
 

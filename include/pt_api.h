@@ -307,6 +307,9 @@ typedef struct pt_trace_stats {
     float shadow_ms;             /* summed HIP-event time of the shadow kernels */
     float shade_ms;
     float total_ms;              /* whole trace calls */
+    float extend_busy_ms;        /* time at least one extend kernel was running: the union of their
+                                    event intervals (launches on two lanes may overlap, so this is at
+                                    most the elapsed time, while extend_ms may exceed it) */
 } pt_trace_stats;
 /* level 0 off, 1 per-kernel event timing, 2 timing + device counters
  * (counting kernels; synchronises once per chunk), 3 event timing of the

@@ -875,6 +875,33 @@ void EventLog::sum(float ms[EV_KINDS], uint32_t launches[EV_KINDS]) {
         launches[p.kind] += 1;
     }
 }
+float EventLog::busy(int kind) {
+    // intervals relative to the first usable pair's start event
+    hipEvent_t ref = nullptr;
+    std::vector<std::pair<float, float>> iv;
+    for (auto& p : pairs) {
+        if (p.kind != kind || !p.a || !p.b || hipEventSynchronize(p.b) != hipSuccess) continue;
+        if (!ref) ref = p.a;
+        float t0 = 0.0f, t1 = 0.0f;
+        if (hipEventElapsedTime(&t0, ref, p.a) != hipSuccess || hipEventElapsedTime(&t1, ref, p.b) != hipSuccess) continue;
+        iv.push_back({t0, t1});
+    }
+    std::sort(iv.begin(), iv.end());
+    float total = 0.0f, lo = 0.0f, hi = 0.0f;
+    bool open = false;
+    for (auto& x : iv) {
+        if (open && x.first <= hi) {
+            hi = std::max(hi, x.second);
+            continue;
+        }
+        if (open) total += hi - lo;
+        lo = x.first;
+        hi = x.second;
+        open = true;
+    }
+    if (open) total += hi - lo;
+    return total;
+}
 EventLog::~EventLog() {
     reset();
     for (hipEvent_t e : pool) (void)hipEventDestroy(e);

@@ -440,6 +440,7 @@ pt_status pt_tracer_stats(pt_tracer* t, pt_trace_stats* out) {
     out->shade_ms = ms[EV_SHADE];
     out->shadow_ms = ms[EV_SHADOW];
     out->total_ms = ms[EV_TOTAL];
+    out->extend_busy_ms = t->log.busy(EV_EXTEND);
     return PT_OK;
 }
 

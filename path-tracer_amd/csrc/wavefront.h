@@ -82,6 +82,8 @@ struct EventLog {
     void take(hipEvent_t& a, hipEvent_t& b);
     void reset();  // recycle recorded pairs
     void sum(float ms[EV_KINDS], uint32_t launches[EV_KINDS]);
+    // time covered by at least one interval of class `kind` (the union of its pairs)
+    float busy(int kind);
     ~EventLog();
 };
 

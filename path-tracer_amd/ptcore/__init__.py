@@ -125,7 +125,8 @@ class TraceStats(C.Structure):
     _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("pair_tests", C.c_uint64),
                 ("tri_tests", C.c_uint64), ("shadow_pair_tests", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
                 ("extend_launches", C.c_uint64), ("extend_ms", C.c_float),
-                ("shadow_ms", C.c_float), ("shade_ms", C.c_float), ("total_ms", C.c_float)]
+                ("shadow_ms", C.c_float), ("shade_ms", C.c_float), ("total_ms", C.c_float),
+                ("extend_busy_ms", C.c_float)]
 
 
 class WaveConfig(C.Structure):
