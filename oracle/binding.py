@@ -12,7 +12,8 @@ import sys
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# ORACLE_LIB selects another build of the oracle (a variant with a planted mistake, for checking the tests)
+LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 sys.path.insert(0, os.path.join(os.path.dirname(_HERE), "path-tracer_amd"))
 import ptcore  # noqa: E402  (structs only)
 
@@ -41,6 +42,15 @@ _SIG = [
     ("oracle_ggx_lambda", C.c_float, [_p, C.c_float, C.c_float]),
     ("oracle_sample_vndf", None, [_p, C.c_float, C.c_float, C.c_float, C.c_float, _p]),
     ("oracle_e_fon_exact", C.c_float, [C.c_float, C.c_float]),
+    ("oracle_ggx_d", C.c_float, [_p, C.c_float, C.c_float]),
+    ("oracle_estimate_e_dielectric", None, [_p, C.c_float, C.c_float, C.c_float, _p, C.POINTER(C.c_uint32), _p]),
+    ("oracle_alpha_anisotropic", None, [C.c_float, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("oracle_evaluate_skybox", None, [C.POINTER(ptcore.SceneDesc), _p, _p]),
+    ("oracle_srgb_decode", C.c_float, [C.c_uint32]),
+    ("oracle_orthonormal_tangent", None, [_p, _p, _p]),
+    ("oracle_sample_atlas", None, [_p, C.c_uint32, C.c_int, C.c_float, C.c_float, C.c_int, _p]),
+    ("oracle_primary_dir", None, [C.POINTER(ptcore.SceneDesc), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.POINTER(C.c_uint32), _p]),
     ("oracle_specular_ior_modulated", C.c_float, [C.c_float, C.c_float]),
     ("oracle_power_heuristic", C.c_float, [C.c_float, C.c_float]),
     ("oracle_evaluate_lobes", None, [_p, _p, _p, _p, _p, _p]),

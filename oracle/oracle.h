@@ -62,11 +62,32 @@ float oracle_fresnel_exact(float ior, float cos_i);
 float oracle_ggx_lambda(const float v[3], float ax, float ay);
 void oracle_sample_vndf(const float wo[3], float ax, float ay, float r, float two_pi_r, float h_out[3]);
 float oracle_e_fon_exact(float mu, float r);
+/* The GGX distribution D(h) of evaluate_lobes (trace.slang:249-250). */
+float oracle_ggx_d(const float h[3], float ax, float ay);
+/* estimate_e_dielectric (trace.slang:396-411) with 1 + Lambda(wo) computed
+ * from wo; draws 32 random numbers from *rng, which it advances. */
+void oracle_estimate_e_dielectric(const float wo[3], float ax, float ay, float specular_ior_modulated,
+                                  const float specular_color[3], uint32_t* rng, float out[3]);
+/* alpha_anisotropic (trace.slang:369-375). */
+void oracle_alpha_anisotropic(float roughness, float anisotropy, float* ax, float* ay);
+/* evaluate_skybox: bilinear sky fetch times average luminance (trace.slang:473-475). */
+void oracle_evaluate_skybox(const pt_scene_desc* scene, const float dir[3], float out[3]);
+/* The sRGB decode table (Rgba8UnormSrgb) the oracle's textures use. */
+float oracle_srgb_decode(uint32_t byte);
+/* orthonormal_tangent (trace.slang:207-218). */
+void oracle_orthonormal_tangent(const float tangent[3], const float normal[3], float out[3]);
+/* The atlas fetch: nearest texel, clamp-to-edge, LOD 0 (world_resources.rs:447-450). */
+void oracle_sample_atlas(const uint8_t* atlases, uint32_t size, int layer, float u, float v, int srgb, float out[4]);
+/* The primary ray direction of pixel (px, py) (trace.slang:726-748); draws
+ * the two jitter numbers from *rng. */
+void oracle_primary_dir(const pt_scene_desc* scene, uint32_t px, uint32_t py, uint32_t iw, uint32_t ih,
+                        uint32_t* rng, float out[3]);
 float oracle_specular_ior_modulated(float ior, float weight);
 float oracle_power_heuristic(float pdf, float other);
 /* evaluate_lobes for a material (base_color rgb, base_weight, metalness,
  * diffuse roughness, specular rgb, specular weight, specular_ior,
- * spec roughness, anisotropy) at wo, wi; E_dielectric given. Writes f[3], pdf. */
+ * spec roughness, spec roughness anisotropy) at wo, wi; E_dielectric given.
+ * Writes f[3], pdf. */
 void oracle_evaluate_lobes(const float params[13], const float wo[3], const float wi[3],
                            const float e_dielectric[3], float f_out[3], float* pdf_out);
 /* Sample a BSDF direction like the bounce does (rands r0 = two_pi_rand/2pi,

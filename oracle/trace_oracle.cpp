@@ -229,14 +229,18 @@ struct Kernel {
         T P_u, P_c, p_ggx, p_diffuse;
     };
 
+    // GGX distribution of evaluate_lobes, trace.slang:249-250
+    T ggx_d(V3<T> H, T ax, T ay) {
+        T D_helper = (H.x * H.x) / (ax * ax) + (H.y * H.y) / (ay * ay) + H.z * H.z;
+        return F(0.31830988618) / (ax * ay * D_helper * D_helper);
+    }
     // trace.slang:229-315
     void evaluate_lobes(const Shading& sh, V3<T> wo, V3<T> wi, V3<T>& f_total, T& pdf_total) {
         V3<T> H = normalize(wo + wi);
         T HdotW = dot(H, wo);
         T G1 = T(1) / sh.one_plus_lambda_wo;
         T G2 = T(1) / (sh.one_plus_lambda_wo + ggx_lambda(wi, sh.ax, sh.ay));
-        T D_helper = (H.x * H.x) / (sh.ax * sh.ax) + (H.y * H.y) / (sh.ay * sh.ay) + H.z * H.z;
-        T D = F(0.31830988618) / (sh.ax * sh.ay * D_helper * D_helper);
+        T D = ggx_d(H, sh.ax, sh.ay);
         T ts = D * G2 / (F(4.0) * wi.z * wo.z);
         T omh = T(1) - HdotW;
         T k = F(0.00105024005) * HdotW * omh;
@@ -1003,6 +1007,51 @@ void oracle_sample_vndf(const float wo[3], float ax, float ay, float r, float tw
     h_out[1] = h.y;
     h_out[2] = h.z;
 }
+float oracle_ggx_d(const float h[3], float ax, float ay) {
+    return Kernel<float>(g_empty_scene, g_dummy).ggx_d({h[0], h[1], h[2]}, ax, ay);
+}
+void oracle_estimate_e_dielectric(const float wo[3], float ax, float ay, float sim, const float spec[3], uint32_t* rng,
+                                  float out[3]) {
+    Rng r{*rng};
+    V3<float> e = Kernel<float>(g_empty_scene, g_dummy)
+                      .estimate_e_dielectric({wo[0], wo[1], wo[2]}, 1.0f + Kernel<float>(g_empty_scene, g_dummy).ggx_lambda(
+                                                                               {wo[0], wo[1], wo[2]}, ax, ay),
+                                             ax, ay, sim, {spec[0], spec[1], spec[2]}, r);
+    *rng = r.state;
+    out[0] = e.x;
+    out[1] = e.y;
+    out[2] = e.z;
+}
+void oracle_alpha_anisotropic(float roughness, float anisotropy, float* ax, float* ay) {
+    Kernel<float>(g_empty_scene, g_dummy).alpha_anisotropic(roughness, anisotropy, *ax, *ay);
+}
+void oracle_evaluate_skybox(const pt_scene_desc* s, const float dir[3], float out[3]) {
+    Counters c;
+    V3<float> v = Kernel<float>(*s, c).evaluate_skybox({dir[0], dir[1], dir[2]});
+    out[0] = v.x;
+    out[1] = v.y;
+    out[2] = v.z;
+}
+float oracle_srgb_decode(uint32_t byte) { return srgb_lut[byte & 255u]; }
+void oracle_orthonormal_tangent(const float t[3], const float n[3], float out[3]) {
+    V3<float> v = Kernel<float>(g_empty_scene, g_dummy).orthonormal_tangent({t[0], t[1], t[2]}, {n[0], n[1], n[2]});
+    out[0] = v.x;
+    out[1] = v.y;
+    out[2] = v.z;
+}
+void oracle_sample_atlas(const uint8_t* atlases, uint32_t size, int layer, float u, float v, int srgb, float out[4]) {
+    sample_atlas<float>(atlases, size, layer, u, v, srgb != 0, out);
+}
+void oracle_primary_dir(const pt_scene_desc* s, uint32_t px, uint32_t py, uint32_t iw, uint32_t ih, uint32_t* rng,
+                        float out[3]) {
+    Counters c;
+    Rng r{*rng};
+    V3<float> d = Kernel<float>(*s, c).primary_dir(px, py, iw, ih, r);
+    *rng = r.state;
+    out[0] = d.x;
+    out[1] = d.y;
+    out[2] = d.z;
+}
 float oracle_e_fon_exact(float mu, float r) { return Kernel<float>(g_empty_scene, g_dummy).e_fon_exact(mu, r); }
 float oracle_specular_ior_modulated(float ior, float weight) {
     return Kernel<float>(g_empty_scene, g_dummy).specular_ior_modulated(ior, weight);
@@ -1011,8 +1060,8 @@ float oracle_power_heuristic(float pdf, float other) {
     return Kernel<float>(g_empty_scene, g_dummy).power_heuristic(pdf, other);
 }
 
-// params: base rgb(3), base_weight, metalness, diffuse_roughness, spec rgb(3), spec_weight, spec_ior, spec_roughness
-// (anisotropy taken as 0)
+// params: base rgb(3), base_weight, metalness, diffuse_roughness, spec rgb(3), spec_weight, spec_ior,
+// spec_roughness, spec_roughness_anisotropy
 static Kernel<float>::Shading make_shading(const float prm[13], V3<float> wo, const float e[3]) {
     Kernel<float> k(g_empty_scene, g_dummy);
     Kernel<float>::Shading sh;
