@@ -523,6 +523,57 @@ def bench_interactive(args):
     return out
 
 
+def trace_union(trace_dir, bench_json):
+    """From a rocprofv3 --kernel-trace of `bench.py --workload W` (CSV output
+    in trace_dir) and that run's JSON line: the union of the traversal
+    launches' busy intervals per timed step and the one-lane launch time, as
+    the kernel trace measures them, beside the bench's own HIP-event figures.
+    The non-counting k_trace dispatches come in time order as the warmup,
+    timed and breakdown jobs on two lanes, then the one-lane jobs. No GPU."""
+    import csv
+
+    with open(bench_json) as f:
+        bench = json.loads(f.read().strip().splitlines()[-1])
+    steps, warm = bench["steps"], bench["warmup"]
+    extra = min(steps, 3)
+    roof = bench["roofline"]
+    l2, l1 = round(roof["launches_per_step"]), round(roof["one_lane"]["launches_per_step"])
+    with open(os.path.join(trace_dir, "run_kernel_trace.csv")) as f:
+        rows = [r for r in csv.DictReader(f) if r["Kernel_Name"].split("(")[0].endswith(", false, false>")
+                and "k_trace<" in r["Kernel_Name"]]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    iv = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in rows]
+    if len(iv) != (warm + steps + extra) * l2 + extra * l1:
+        raise SystemExit(f"unexpected traversal dispatch count {len(iv)}")
+
+    def union(x):
+        x = sorted(x)
+        tot, lo, hi = 0, None, None
+        for a, b in x:
+            if hi is not None and a <= hi:
+                hi = max(hi, b)
+                continue
+            if hi is not None:
+                tot += hi - lo
+            lo, hi = a, b
+        return tot + (hi - lo if hi is not None else 0)
+
+    timed = [iv[(warm + k) * l2:(warm + k + 1) * l2] for k in range(steps)]
+    one = iv[(warm + steps + extra) * l2:]
+    tc = bench["traversal_counts_per_step"]
+    bytes_step = PAIR_BYTES * (tc["pair_tests"] + tc["shadow_pair_tests"]) + TRI_BYTES * (tc["tri_tests"] + tc["shadow_tri_tests"])
+    busy = sum(union(j) for j in timed) / steps / 1e6
+    one_ms = sum(b - a for a, b in one) / len(one) / 1e6
+    return {"dispatches": len(iv), "launches_per_two_lane_job": l2, "launches_per_one_lane_job": l1,
+            "algorithmic_bytes_per_step": bytes_step,
+            "trace_busy_ms_per_step": busy, "trace_frac": bytes_step / (busy * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "bench_busy_ms_per_step": roof["busy_ms_per_step"], "bench_frac": roof["frac"],
+            "trace_per_launch_mean_ms": sum(b - a for j in timed for a, b in j) / (steps * l2) / 1e6,
+            "trace_one_lane_ms": one_ms,
+            "trace_one_lane_frac": bytes_step / l1 / (one_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "bench_one_lane_ms": roof["one_lane"]["avg_launch_ms"], "bench_one_lane_frac": roof["one_lane"]["frac"]}
+
+
 def headline_line(m, args, ctx):
     import ptcore
 
@@ -565,7 +616,12 @@ def main():
     ap.add_argument("--chunk", type=int, default=None, help="paths per wavefront chunk (default: library default)")
     ap.add_argument("--streams", type=int, default=None, choices=[0, 1, 2],
                     help="wavefront chunk lanes on separate streams (default: library default)")
+    ap.add_argument("--trace-union", nargs=2, metavar=("TRACE_DIR", "BENCH_JSON"),
+                    help="no GPU: the traversal busy time of a rocprofv3 kernel trace of this bench")
     args = ap.parse_args()
+    if args.trace_union:
+        print(json.dumps(trace_union(*args.trace_union)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(pixel/tile sharding across GPUs is designed but not built)", file=sys.stderr)
