@@ -74,6 +74,41 @@ void pt_image_free(uint8_t* data);
 /* image::save_buffer: encode by extension and write the file. */
 pt_status pt_image_save(const char* path, const uint8_t* rgb, uint32_t width, uint32_t height);
 
+/* ------------------------------------------------------------- scene export */
+/* Write a world's input (what pt_world_build takes) as a glTF binary that
+ * the reference's Scene::new (src/render/world.rs:887-1423) and
+ * pt_assets_load_scene read back into the same triangles, materials and
+ * atlases, and (sky_path may be NULL) its sky as an uncompressed FLOAT
+ * OpenEXR that Skybox::new (world.rs:1872-1873) and pt_assets_load_skybox
+ * read back exactly; a sky_path ending in ".hdr" writes Radiance RGBE
+ * instead, and fails (PT_ERR_INVALID) unless every sky value is exactly an
+ * RGBE value (c * 2^(e - 136), alpha 1).
+ * One node without a transform holds one mesh with a primitive per run of
+ * triangles of one material, with explicit NORMAL, TANGENT and TEXCOORD_0
+ * per corner; textures become RGBA PNG images. Each normal and tangent
+ * direction is written as a vector that the loader's normalisation turns
+ * back into the stored one; where none lies within 3 ulps it is written as
+ * stored and loads back within a few ulps, and *inexact_directions (may be
+ * NULL) counts those (0 for worlds from pt_generate_scene and
+ * pt_assets_load_scene). Materials not numbered in order of first use are
+ * written but load renumbered. PT_ERR_INVALID for what the format cannot
+ * hold: atlas layers other than 8192^2, texture rectangles the loader's
+ * packer would place elsewhere, OpenPBR parameters outside [0, 1] and a
+ * normal scale without a normal texture. The camera and the sky luminance
+ * are render flags, not file contents. */
+pt_status pt_world_export_glb(const pt_world_desc* desc, const char* glb_path, const char* sky_path,
+                              uint32_t* inexact_directions);
+
+/* The device-layout arrays of a scene (pt_scene_desc, BVH included) as a
+ * .ptscene file, and back. No counterpart in the reference: it lets a tool
+ * hand over the identical serialized BVH. A loaded file owns its arrays;
+ * pt_scene_file_desc borrows them until pt_scene_file_destroy. */
+typedef struct pt_scene_file pt_scene_file;
+pt_status pt_scene_save(const pt_scene_desc* desc, const char* path);
+pt_status pt_scene_file_load(const char* path, pt_scene_file** out);
+pt_status pt_scene_file_desc(const pt_scene_file* file, pt_scene_desc* out);
+pt_status pt_scene_file_destroy(pt_scene_file* file);
+
 #ifdef __cplusplus
 }
 #endif

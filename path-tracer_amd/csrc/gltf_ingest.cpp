@@ -35,14 +35,13 @@
 #include <vector>
 
 #include "../../include/pt_assets.h"
+#include "gltf_common.h"
 #include "image_io.h"
 #include "json_lite.h"
 #include "pt_internal.h"
 #include "tangents.h"
 
 namespace ptcore {
-
-constexpr uint32_t ATLAS = 8192;
 
 struct Assets {
     std::vector<PtTriangleGeometry> geometries;
@@ -132,10 +131,6 @@ void point3(const M4& m, const float p[3], float o[3]) {
 }
 void vector3(const M4& m, const float v[3], float o[3]) {
     for (int r = 0; r < 3; ++r) o[r] = (m.c[0][r] * v[0] + m.c[1][r] * v[1]) + m.c[2][r] * v[2];
-}
-void normalize3(float v[3]) {
-    const float inv = 1.0f / std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
-    for (int k = 0; k < 3; ++k) v[k] *= inv;
 }
 
 // ------------------------------------------------------------- buffers and accessors
@@ -285,35 +280,6 @@ bool accessor(const Doc& d, long ai, Accessor& a, std::string& err, int want_nco
     a.base = v.p + (size_t)off;
     return true;
 }
-
-// ------------------------------------------------------------- atlas packing
-struct Rect {
-    uint32_t x, y, w, h;
-};
-struct Layer {
-    std::vector<Rect> free{{0, 0, ATLAS, ATLAS}};
-    bool place(uint32_t w, uint32_t h, uint32_t& x, uint32_t& y) {
-        // best short-side fit among free rectangles, then a guillotine split
-        size_t best = free.size();
-        uint32_t best_fit = std::numeric_limits<uint32_t>::max();
-        for (size_t i = 0; i < free.size(); ++i)
-            if (free[i].w >= w && free[i].h >= h) {
-                uint32_t fit = std::min(free[i].w - w, free[i].h - h);
-                if (fit < best_fit) {
-                    best_fit = fit;
-                    best = i;
-                }
-            }
-        if (best == free.size()) return false;
-        Rect r = free[best];
-        free.erase(free.begin() + (long)best);
-        x = r.x;
-        y = r.y;
-        if (r.w > w) free.push_back({r.x + w, r.y, r.w - w, h});
-        if (r.h > h) free.push_back({r.x, r.y + h, r.w, r.h - h});
-        return true;
-    }
-};
 
 // ------------------------------------------------------------- materials
 struct ImageSlot {
@@ -504,7 +470,7 @@ bool load_scene(const uint8_t* data, size_t size, Assets& A, std::string& err) {
             const Json* spec = ext ? ext->get("KHR_materials_specular") : nullptr;
             note_srgb(spec ? spec->get("specularColorTexture") : nullptr);
         }
-    std::vector<Layer> srgb_layers, lin_layers;
+    std::vector<AtlasLayer> srgb_layers, lin_layers;
     std::map<long, ImageSlot> srgb_slots, lin_slots;
     if (const Json* images = d.j.get("images")) {
         for (size_t ii = 0; ii < images->arr.size(); ++ii) {
@@ -532,7 +498,7 @@ bool load_scene(const uint8_t* data, size_t size, Assets& A, std::string& err) {
             std::vector<uint8_t> rgba;
             to_rgba8(img, rgba);
             const bool is_srgb = srgb_images.count((long)ii) != 0;
-            std::vector<Layer>& layers = is_srgb ? srgb_layers : lin_layers;
+            std::vector<AtlasLayer>& layers = is_srgb ? srgb_layers : lin_layers;
             std::vector<uint8_t>& pixels = is_srgb ? A.srgb : A.linear;
             uint32_t x = 0, y = 0;
             size_t li = 0;

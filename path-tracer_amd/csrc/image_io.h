@@ -37,4 +37,14 @@ bool decode_exr(const uint8_t* data, size_t size, FloatImage& out, std::string& 
 // A sky image as image::load_from_memory(..).into_rgba32f() gives it (Radiance .hdr, OpenEXR, PNG or JPEG).
 bool decode_sky_image(const uint8_t* data, size_t size, FloatImage& out, std::string& err);
 
+// PNG of 8-bit rows with 3 (RGB) or 4 (RGBA) channels.
+bool encode_png_channels(const uint8_t* pixels, size_t stride, uint32_t w, uint32_t h, int channels,
+                         std::vector<uint8_t>& out, std::string& err);
+// Uncompressed scanline OpenEXR with FLOAT R, G, B, A channels (lossless for RGBA32F).
+bool encode_exr_float(const float* rgba, uint32_t w, uint32_t h, std::vector<uint8_t>& out, std::string& err);
+// Radiance .hdr (run-length coded RGBE scanlines) of an image whose every value
+// the RGBE decoding c * 2^(E - 136) gives back exactly and whose alpha is 1;
+// fails otherwise.
+bool encode_hdr_exact(const float* rgba, uint32_t w, uint32_t h, std::vector<uint8_t>& out, std::string& err);
+
 }  // namespace ptcore

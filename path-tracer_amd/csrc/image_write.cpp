@@ -55,9 +55,13 @@ int paeth(int a, int b, int c) {
     return pb <= pc ? b : c;
 }
 
-bool encode_png(const uint8_t* rgb, size_t stride, uint32_t w, uint32_t h, std::vector<uint8_t>& out,
-                std::string& err) {
-    const size_t row = (size_t)w * 3;
+}  // namespace
+
+// PNG of 8-bit RGB (channels 3) or RGBA (channels 4) rows
+bool encode_png_channels(const uint8_t* rgb, size_t stride, uint32_t w, uint32_t h, int channels,
+                         std::vector<uint8_t>& out, std::string& err) {
+    const int C = channels == 4 ? 4 : 3;
+    const size_t row = (size_t)w * C;
     std::vector<uint8_t> raw((row + 1) * h), cand(row);
     std::vector<uint8_t> zero(row, 0);
     for (uint32_t y = 0; y < h; ++y) {
@@ -68,7 +72,7 @@ bool encode_png(const uint8_t* rgb, size_t stride, uint32_t w, uint32_t h, std::
         for (int f = 0; f < 5; ++f) {
             uint64_t cost = 0;
             for (size_t i = 0; i < row; ++i) {
-                int a = i >= 3 ? cur[i - 3] : 0, b = prev[i], c = i >= 3 ? prev[i - 3] : 0;
+                int a = i >= (size_t)C ? cur[i - C] : 0, b = prev[i], c = i >= (size_t)C ? prev[i - C] : 0;
                 int pred = f == 0 ? 0 : f == 1 ? a : f == 2 ? b : f == 3 ? (a + b) >> 1 : paeth(a, b, c);
                 uint8_t v = (uint8_t)(cur[i] - pred);
                 cand[i] = v;
@@ -100,11 +104,149 @@ bool encode_png(const uint8_t* rgb, size_t stride, uint32_t w, uint32_t h, std::
     std::vector<uint8_t> ihdr;
     put_be32(ihdr, w);
     put_be32(ihdr, h);
-    ihdr.insert(ihdr.end(), {8, 2, 0, 0, 0});  // 8-bit truecolour, deflate, adaptive filters, no interlace
+    // 8-bit truecolour (with alpha for 4 channels), deflate, adaptive filters, no interlace
+    ihdr.insert(ihdr.end(), {8, (uint8_t)(C == 4 ? 6 : 2), 0, 0, 0});
     chunk("IHDR", ihdr.data(), ihdr.size());
     chunk("IDAT", z.data(), z.size());
     chunk("IEND", nullptr, 0);
     return true;
+}
+
+// Single-part scanline OpenEXR, uncompressed, FLOAT channels A, B, G, R (the
+// channel list is sorted by name), one scanline per block.
+bool encode_exr_float(const float* rgba, uint32_t w, uint32_t h, std::vector<uint8_t>& out, std::string& err) {
+    if (!w || !h) {
+        err = "EXR: empty image";
+        return false;
+    }
+    out.clear();
+    auto u32 = [&](uint32_t v) { put_le32(out, v); };
+    auto str = [&](const char* t) { out.insert(out.end(), t, t + std::strlen(t) + 1); };
+    auto attr = [&](const char* name, const char* type, uint32_t size) {
+        str(name);
+        str(type);
+        u32(size);
+    };
+    auto f32 = [&](float f) {
+        uint32_t b;
+        std::memcpy(&b, &f, 4);
+        u32(b);
+    };
+    u32(20000630u);  // magic
+    u32(2u);         // version 2, scanline, single part
+    attr("channels", "chlist", 4 * (2 + 16) + 1);
+    for (const char* ch : {"A", "B", "G", "R"}) {
+        str(ch);
+        u32(2u);  // FLOAT
+        out.insert(out.end(), {0, 0, 0, 0});  // pLinear, reserved
+        u32(1u);
+        u32(1u);  // x and y sampling
+    }
+    out.push_back(0);
+    attr("compression", "compression", 1);
+    out.push_back(0);  // NO_COMPRESSION
+    attr("dataWindow", "box2i", 16);
+    u32(0), u32(0), u32(w - 1), u32(h - 1);
+    attr("displayWindow", "box2i", 16);
+    u32(0), u32(0), u32(w - 1), u32(h - 1);
+    attr("lineOrder", "lineOrder", 1);
+    out.push_back(0);  // INCREASING_Y
+    attr("pixelAspectRatio", "float", 4);
+    f32(1.0f);
+    attr("screenWindowCenter", "v2f", 8);
+    f32(0.0f), f32(0.0f);
+    attr("screenWindowWidth", "float", 4);
+    f32(1.0f);
+    out.push_back(0);  // end of header
+    const size_t table = out.size();
+    out.resize(table + (size_t)h * 8);
+    const uint32_t line_bytes = w * 4 * 4;
+    for (uint32_t y = 0; y < h; ++y) {
+        const uint64_t at = out.size();
+        std::memcpy(&out[table + (size_t)y * 8], &at, 8);
+        u32(y);
+        u32(line_bytes);
+        for (int c : {3, 2, 1, 0})  // A, B, G, R
+            for (uint32_t x = 0; x < w; ++x) f32(rgba[((size_t)y * w + x) * 4 + c]);
+    }
+    return true;
+}
+
+static bool hdr_fail(std::string& err, const char* m) {
+    err = std::string("HDR: ") + m;
+    return false;
+}
+
+bool encode_hdr_exact(const float* rgba, uint32_t w, uint32_t h, std::vector<uint8_t>& out, std::string& err) {
+    if (w == 0 || h == 0 || w > 65535 || h > 65535) return hdr_fail(err, "bad HDR size");
+    char head[96];
+    std::snprintf(head, sizeof(head), "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %u +X %u\n", h, w);
+    out.assign(head, head + std::strlen(head));
+    std::vector<uint8_t> scan((size_t)w * 4);
+    for (uint32_t y = 0; y < h; ++y) {
+        for (uint32_t x = 0; x < w; ++x) {
+            const float* p = &rgba[((size_t)y * w + x) * 4];
+            uint8_t* e = &scan[(size_t)x * 4];
+            const float mx = std::fmax(p[0], std::fmax(p[1], p[2]));
+            e[0] = e[1] = e[2] = e[3] = 0;
+            if (mx > 0.0f) {
+                int ex;
+                std::frexp(mx, &ex);  // mx = f * 2^ex, f in [0.5, 1)
+                if (ex + 128 < 1 || ex + 128 > 255) return hdr_fail(err, "sky value outside the RGBE exponent range");
+                for (int c = 0; c < 3; ++c) e[c] = (uint8_t)(int)std::ldexp(p[c], 8 - ex);
+                e[3] = (uint8_t)(ex + 128);
+            }
+            // exact only if the decoder (c * 2^(E - 136)) gives every value back
+            for (int c = 0; c < 3; ++c) {
+                const float back = e[3] ? std::ldexp((float)e[c], (int)e[3] - 136) : 0.0f;
+                if (!(back == p[c]) || std::signbit(p[c]))
+                    return hdr_fail(err, "sky is not exactly representable in RGBE (write it as .exr)");
+            }
+            if (p[3] != 1.0f) return hdr_fail(err, "sky alpha is not 1 (RGBE holds no alpha; write it as .exr)");
+        }
+        const bool rle = w >= 8 && w < 32768;
+        if (!rle) {
+            out.insert(out.end(), scan.begin(), scan.end());
+            continue;
+        }
+        out.push_back(2);
+        out.push_back(2);
+        out.push_back((uint8_t)(w >> 8));
+        out.push_back((uint8_t)(w & 0xFF));
+        for (int c = 0; c < 4; ++c) {  // each component run-length coded on its own
+            uint32_t x = 0;
+            while (x < w) {
+                uint32_t run = 1;
+                while (x + run < w && run < 127 && scan[(size_t)(x + run) * 4 + c] == scan[(size_t)x * 4 + c]) ++run;
+                if (run >= 3) {
+                    out.push_back((uint8_t)(128 + run));
+                    out.push_back(scan[(size_t)x * 4 + c]);
+                    x += run;
+                    continue;
+                }
+                // literal span up to the next run of 3 or 128 values
+                uint32_t n = 0;
+                while (x + n < w && n < 128) {
+                    const uint32_t i = x + n;
+                    if (i + 2 < w && scan[(size_t)i * 4 + c] == scan[(size_t)(i + 1) * 4 + c] &&
+                        scan[(size_t)i * 4 + c] == scan[(size_t)(i + 2) * 4 + c])
+                        break;
+                    ++n;
+                }
+                out.push_back((uint8_t)n);
+                for (uint32_t k = 0; k < n; ++k) out.push_back(scan[(size_t)(x + k) * 4 + c]);
+                x += n;
+            }
+        }
+    }
+    return true;
+}
+
+namespace {
+
+bool encode_png(const uint8_t* rgb, size_t stride, uint32_t w, uint32_t h, std::vector<uint8_t>& out,
+                std::string& err) {
+    return encode_png_channels(rgb, stride, w, h, 3, out, err);
 }
 
 // ---------------------------------------------------------------- JPEG

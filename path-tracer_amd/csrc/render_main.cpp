@@ -10,8 +10,13 @@
 // extension names (.png, .jpg/.jpeg, .webp). --scene takes a glTF file
 // (.glb, or .gltf with embedded data) with --skybox <.hdr|.exr|.png>, loaded by the
 // asset front end (pt_assets.h), or one of the procedural generators
-// (cornell | field | large | sized:<triangles>). Extra flags: --seed,
-// --device, --schedule mega|wavefront, --pfm <file> (linear storage image dump).
+// (cornell | field | large | sized:<triangles>), or a .ptscene file (the
+// built arrays, camera and sky baked in, so the camera and sky flags do not
+// apply). Extra flags: --seed, --device, --schedule mega|wavefront, --pfm
+// <file> (linear storage image dump), and --export-glb <file.glb>
+// [--export-sky <file.exr|.hdr>] / --export-ptscene <file>, which write the
+// world (pt_world_export_glb, pt_scene_save), print the render flags that
+// give the same camera and sky luminance, and exit without rendering.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -64,6 +69,7 @@ void write_pfm(const std::string& path, const std::vector<float>& rgb, uint32_t 
 
 int main(int argc, char** argv) {
     std::string scene = "cornell", skybox, output = "render.png", pfm, schedule = "wavefront";
+    std::string export_glb, export_sky, export_ptscene;
     uint32_t width = 1920, height = 1080, samples = 256, max_bounces = PT_DEFAULT_MAX_BOUNCES, tile = 512, device = 0;
     uint64_t seed = 1;
     float sky_lum = -1.0f;
@@ -85,6 +91,9 @@ int main(int argc, char** argv) {
             check(pt_image_format_from_path(output.c_str(), &f), "--output");
         }
         else if (a == "--pfm") pfm = val();
+        else if (a == "--export-glb") export_glb = val();
+        else if (a == "--export-sky") export_sky = val();
+        else if (a == "--export-ptscene") export_ptscene = val();
         else if (a == "--width") width = parse_u32(val(), "--width", 1, 1u << 16);
         else if (a == "--height") height = parse_u32(val(), "--height", 1, 1u << 16);
         else if (a == "--samples") samples = parse_u32(val(), "--samples", 1, 1u << 24);
@@ -119,7 +128,8 @@ int main(int argc, char** argv) {
                         "  [--width W] [--height H]\n"
                         "  [--samples S] [--max-bounces D] [--tile-size T] [--output out.png|.jpg|.webp] [--pfm out.pfm]\n"
                         "  [--skybox-luminance L] [--camera-position X,Y,Z] [--camera-yaw DEG]\n"
-                        "  [--camera-pitch DEG] [--camera-fov DEG] [--seed N] [--schedule mega|wavefront]\n");
+                        "  [--camera-pitch DEG] [--camera-fov DEG] [--seed N] [--schedule mega|wavefront]\n"
+                        "  [--export-glb W.glb [--export-sky S.exr|S.hdr]] [--export-ptscene F.ptscene]\n");
             return 0;
         } else die("unknown argument %s", a.c_str());
     }
@@ -128,10 +138,22 @@ int main(int argc, char** argv) {
         const size_t n = std::strlen(suffix);
         return scene.size() >= n && scene.compare(scene.size() - n, n, suffix) == 0;
     };
+    if (!export_sky.empty() && export_glb.empty()) die("--export-sky goes with --export-glb");
+    const bool exporting = !export_glb.empty() || !export_ptscene.empty();
     pt_generated* gen = nullptr;
     pt_assets* assets = nullptr;
+    pt_scene_file* baked = nullptr;
+    pt_world* world = nullptr;
     pt_world_desc wd;
-    if (ends_with(".glb") || ends_with(".gltf")) {
+    pt_scene_desc sd;
+    if (ends_with(".ptscene")) {
+        if (!skybox.empty() || sky_lum >= 0.0f || !std::isnan(cam_pos[0]) || !std::isnan(yaw) || !std::isnan(pitch) ||
+            !std::isnan(fov))
+            die("a .ptscene holds its camera and sky: --skybox, --skybox-luminance and the camera flags do not apply");
+        if (!export_glb.empty()) die("--export-glb takes a world (a glTF file or a generator), not a .ptscene");
+        check(pt_scene_file_load(scene.c_str(), &baked), "load .ptscene");
+        check(pt_scene_file_desc(baked, &sd), ".ptscene arrays");
+    } else if (ends_with(".glb") || ends_with(".gltf")) {
         if (skybox.empty()) die("--skybox <file> is required with a glTF scene");
         check(pt_assets_create(&assets), "assets");
         check(pt_assets_load_scene_file(assets, scene.c_str()), "load scene");
@@ -157,16 +179,44 @@ int main(int argc, char** argv) {
         check(pt_generated_world_desc(gen, &wd), "scene description");
         if (sky_lum >= 0.0f) wd.skybox_luminance = sky_lum;
     }
-    if (!std::isnan(cam_pos[0]))
-        for (int k = 0; k < 3; ++k) wd.camera_position[k] = cam_pos[k];
-    if (!std::isnan(yaw)) wd.camera_yaw_degrees = yaw;
-    if (!std::isnan(pitch)) wd.camera_pitch_degrees = pitch;
-    if (!std::isnan(fov)) wd.camera_fov_degrees = fov;
-    wd.aspect_ratio = (float)width / (float)height;
-    pt_world* world = nullptr;
-    check(pt_world_build(&wd, &world), "build world");
-    pt_scene_desc sd;
-    check(pt_world_scene_desc(world, &sd), "scene arrays");
+    if (!baked) {
+        if (!std::isnan(cam_pos[0]))
+            for (int k = 0; k < 3; ++k) wd.camera_position[k] = cam_pos[k];
+        if (!std::isnan(yaw)) wd.camera_yaw_degrees = yaw;
+        if (!std::isnan(pitch)) wd.camera_pitch_degrees = pitch;
+        if (!std::isnan(fov)) wd.camera_fov_degrees = fov;
+        wd.aspect_ratio = (float)width / (float)height;
+        if (!export_glb.empty()) {
+            uint32_t inexact = 0;
+            check(pt_world_export_glb(&wd, export_glb.c_str(), export_sky.empty() ? nullptr : export_sky.c_str(), &inexact),
+                  "export glTF");
+            std::printf("Wrote %s%s%s (%u normal/tangent directions inexact)\n", export_glb.c_str(),
+                        export_sky.empty() ? "" : " and ", export_sky.c_str(), inexact);
+            // the camera and sky luminance are render flags (headless.rs:14-64)
+            std::printf("render flags:");
+            if (!std::isnan(wd.camera_position[0]))
+                std::printf(" --camera-position %.9g,%.9g,%.9g", wd.camera_position[0], wd.camera_position[1],
+                            wd.camera_position[2]);
+            if (!std::isnan(wd.camera_yaw_degrees)) std::printf(" --camera-yaw %.9g", wd.camera_yaw_degrees);
+            if (!std::isnan(wd.camera_pitch_degrees)) std::printf(" --camera-pitch %.9g", wd.camera_pitch_degrees);
+            if (!std::isnan(wd.camera_fov_degrees)) std::printf(" --camera-fov %.9g", wd.camera_fov_degrees);
+            if (wd.skybox_luminance >= 0.0f) std::printf(" --skybox-luminance %.9g", wd.skybox_luminance);
+            std::printf(" --width %u --height %u\n", width, height);
+        }
+        check(pt_world_build(&wd, &world), "build world");
+        check(pt_world_scene_desc(world, &sd), "scene arrays");
+    }
+    if (!export_ptscene.empty()) {
+        check(pt_scene_save(&sd, export_ptscene.c_str()), "save .ptscene");
+        std::printf("Wrote %s (%u triangles, %u BVH nodes)\n", export_ptscene.c_str(), sd.triangle_count, sd.bvh_node_count);
+    }
+    if (exporting) {
+        pt_world_destroy(world);
+        pt_generated_destroy(gen);
+        pt_assets_destroy(assets);
+        pt_scene_file_destroy(baked);
+        return 0;
+    }
     std::printf("Loaded scene %s with %u triangles, %u materials, %u BVH nodes\n", scene.c_str(), sd.triangle_count,
                 sd.material_count, sd.bvh_node_count);
 
@@ -177,6 +227,7 @@ int main(int argc, char** argv) {
     pt_world_destroy(world);  // the GPU has its own copy (headless.rs:157)
     pt_generated_destroy(gen);
     pt_assets_destroy(assets);
+    pt_scene_file_destroy(baked);
 
     const uint32_t tw = std::min(tile, width), th = std::min(tile, height);
     pt_tracer* tracer = nullptr;

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/pt_scenes.h"
+#include "gltf_common.h"
 #include "pt_internal.h"
 
 namespace {
@@ -83,6 +84,7 @@ struct pt_generated {
     float sky_luminance = 1.0f;
     float cam_pos[3] = {NAN, NAN, NAN};
     float yaw = NAN, pitch = NAN, fov = NAN, aspect = 1.0f;
+    float tex_uv[4][4] = {};  // atlas offset and scale of the procedural texture pairs
 
     void tri(V a, V b, V c, V na, V nb, V nc, V ta, V tb, V tc, float uv[6], uint32_t mat) {
         PtTriangleGeometry g = {{a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z}};
@@ -378,16 +380,18 @@ void gradient_sky(pt_generated& g, uint32_t w, uint32_t h, bool sun, bool rgbe) 
             }
             float* p = &g.sky[((size_t)y * w + x) * 4];
             if (rgbe) {
-                // quantise through the Radiance RGBE encoding so the values are exactly representable there
+                // quantise to what an RGBE pixel decodes to (mantissa byte q, exponent byte
+                // e + 128: q * 2^(e - 8), as the image crate's Rgbe8Pixel::to_hdr gives it),
+                // so the sky can be written as a Radiance .hdr without loss
                 float mx = std::fmax(c[0], std::fmax(c[1], c[2]));
                 if (mx < 1e-32f) {
                     p[0] = p[1] = p[2] = 0.0f;
                 } else {
                     int e;
-                    float m = std::frexp(mx, &e) * 256.0f / mx;
+                    std::frexp(mx, &e);
                     for (int k = 0; k < 3; ++k) {
-                        int q = (int)(c[k] * m);
-                        p[k] = std::ldexp((float)q + 0.5f, e - 8);
+                        int q = (int)std::ldexp(c[k], 8 - e);
+                        p[k] = std::ldexp((float)q, e - 8);
                     }
                 }
             } else {
@@ -463,9 +467,8 @@ void field_scene(pt_generated& g, Rng64& r, uint32_t grid, uint32_t spheres, int
     if (textured_pairs > 0) {
         ground.base_color_tex_layer = 0;
         ground.metallic_roughness_tex_layer = 0;
-        float uvo[4] = {0.0f, 0.0f, 0.5f, 0.5f};
-        std::memcpy(ground.base_color_uv, uvo, sizeof(uvo));
-        std::memcpy(ground.metallic_roughness_uv, uvo, sizeof(uvo));
+        std::memcpy(ground.base_color_uv, g.tex_uv[0], sizeof(ground.base_color_uv));
+        std::memcpy(ground.metallic_roughness_uv, g.tex_uv[0], sizeof(ground.metallic_roughness_uv));
     }
     g.mats.push_back(ground);
     const uint32_t sphere_mats = std::min<uint32_t>(64, std::max<uint32_t>(1, spheres));
@@ -479,9 +482,8 @@ void field_scene(pt_generated& g, Rng64& r, uint32_t grid, uint32_t spheres, int
             int pair = 1 + (int)(i % 3);
             m.base_color_tex_layer = 0;
             m.metallic_roughness_tex_layer = 0;
-            float uvo[4] = {0.5f * (float)(pair & 1), 0.5f * (float)(pair >> 1), 0.5f, 0.5f};
-            std::memcpy(m.base_color_uv, uvo, sizeof(uvo));
-            std::memcpy(m.metallic_roughness_uv, uvo, sizeof(uvo));
+            std::memcpy(m.base_color_uv, g.tex_uv[pair], sizeof(m.base_color_uv));
+            std::memcpy(m.metallic_roughness_uv, g.tex_uv[pair], sizeof(m.metallic_roughness_uv));
         }
         g.mats.push_back(m);
     }
@@ -500,18 +502,25 @@ void field_scene(pt_generated& g, Rng64& r, uint32_t grid, uint32_t spheres, int
 }
 
 void perlin_atlases(pt_generated& g, Rng64& r, uint32_t tex_size) {
-    // one sRGB layer with 4 albedo textures and one linear layer with 4
-    // metallic-roughness textures, each tex_size^2 in a 2x2 grid
-    const uint32_t A = tex_size * 2;
+    // one sRGB atlas layer with 4 albedo textures and one linear layer with 4
+    // metallic-roughness textures, each tex_size^2, placed where the glTF
+    // loader's packer puts 4 such images in that order; the rest of each
+    // layer is zero, as the loader leaves it
+    const uint32_t A = ptcore::ATLAS;
     g.atlas_size = A;
-    g.srgb.assign((size_t)A * A * 4, 255);
-    g.linear.assign((size_t)A * A * 4, 255);
+    g.srgb.assign((size_t)A * A * 4, 0);
+    g.linear.assign((size_t)A * A * 4, 0);
     g.srgb_count = g.linear_count = 1;
+    ptcore::AtlasLayer packer;
     for (int t = 0; t < 4; ++t) {
         Perlin pa(r), pr(r);
         float base[3] = {r.uniform(0.2f, 0.9f), r.uniform(0.2f, 0.9f), r.uniform(0.2f, 0.9f)};
         float tint[3] = {r.uniform(0.1f, 0.9f), r.uniform(0.1f, 0.9f), r.uniform(0.1f, 0.9f)};
-        const uint32_t ox = (uint32_t)(t & 1) * tex_size, oy = (uint32_t)(t >> 1) * tex_size;
+        uint32_t ox = 0, oy = 0;
+        packer.place(tex_size, tex_size, ox, oy);
+        g.tex_uv[t][0] = (float)ox / (float)A;
+        g.tex_uv[t][1] = (float)oy / (float)A;
+        g.tex_uv[t][2] = g.tex_uv[t][3] = (float)tex_size / (float)A;
         const float freq = 8.0f / (float)tex_size;
         for (uint32_t y = 0; y < tex_size; ++y)
             for (uint32_t x = 0; x < tex_size; ++x) {
@@ -529,6 +538,38 @@ void perlin_atlases(pt_generated& g, Rng64& r, uint32_t tex_size) {
                 l[3] = 255;
             }
     }
+}
+
+// Make the world one that a glTF file can hold exactly: written as .glb with
+// explicit normals and tangents and read back by the scene loader, it gives
+// the same arrays bit for bit.
+// - materials are numbered in order of first use by the triangles, as the
+//   loader creates them (unused ones are dropped);
+// - no coordinate is -0 (the loader's identity transform turns it into +0),
+//   and normals and tangents are fixed points of the loader's normalisation;
+// - a triangle with the tangent (1, 0, 0) and w = 1 at a corner, the value the
+//   loader takes for "no tangents" and replaces by MikkTSpace tangents, gets
+//   the same frame written as tangent (-1, 0, 0) with w = -1 at every corner
+//   (bitangent = cross(n, t) * w is unchanged).
+bool canonicalize(pt_generated& g) {
+    std::vector<int64_t> remap(g.mats.size(), -1);
+    std::vector<PtMaterial> mats;
+    for (PtTriangleAttribute& at : g.attr) {
+        if (remap[at.material] < 0) {
+            remap[at.material] = (int64_t)mats.size();
+            mats.push_back(g.mats[at.material]);
+        }
+        at.material = (uint32_t)remap[at.material];
+    }
+    g.mats.swap(mats);
+    for (PtTriangleGeometry& tri : g.geo)
+        for (float& c : tri.v) c = c + 0.0f;
+    for (PtTriangleAttribute& at : g.attr) {
+        float* t[3] = {at.t0, at.t1, at.t2};
+        for (int k = 0; k < 3; ++k)
+            if (!ptcore::canonical_unit(&at.n[3 * k]) || !ptcore::canonical_unit(t[k])) return false;
+    }
+    return true;
 }
 
 }  // namespace
@@ -597,6 +638,10 @@ pt_status pt_generate_scene(int kind, uint64_t seed, uint32_t triangles, uint32_
     } catch (const std::bad_alloc&) {
         delete g;
         return set_error(PT_ERR_MEMORY, "out of host memory while generating a scene");
+    }
+    if (!canonicalize(*g)) {
+        delete g;
+        return set_error(PT_ERR_STATE, "no load-stable normal near a generated one");
     }
     g->aspect = (float)width / (float)height;
     *out = g;

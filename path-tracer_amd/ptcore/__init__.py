@@ -258,6 +258,11 @@ _SIGNATURES = [
                                C.POINTER(C.c_size_t)]),
     ("pt_image_free", None, [C.POINTER(C.c_uint8)]),
     ("pt_image_save", _i32, [C.c_char_p, _p, _u32, _u32]),
+    ("pt_world_export_glb", _i32, [C.POINTER(WorldDesc), C.c_char_p, C.c_char_p, C.POINTER(_u32)]),
+    ("pt_scene_save", _i32, [C.POINTER(SceneDesc), C.c_char_p]),
+    ("pt_scene_file_load", _i32, [C.c_char_p, C.POINTER(_p)]),
+    ("pt_scene_file_desc", _i32, [_p, C.POINTER(SceneDesc)]),
+    ("pt_scene_file_destroy", _i32, [_p]),
     ("pt_generate_scene", _i32, [C.c_int, C.c_uint64, _u32, _u32, _u32, C.POINTER(_p)]),
     ("pt_generated_world_desc", _i32, [_p, C.POINTER(WorldDesc)]),
     ("pt_generated_destroy", _i32, [_p]),
@@ -297,10 +302,12 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
-def _view(ptr, dtype, count):
+def _view(ptr, dtype, count, owner=None):
+    """A numpy view of library memory; `owner` (the object that frees it) stays alive with the view."""
     if not ptr or count == 0:
         return np.zeros(0, dtype=dtype)
     buf = (C.c_char * (count * np.dtype(dtype).itemsize)).from_address(ptr)
+    buf._owner = owner
     return np.frombuffer(buf, dtype=dtype, count=count)
 
 
@@ -328,13 +335,68 @@ class Generated:
 
     def arrays(self):
         d = self.desc
+        layer = d.atlas_size * d.atlas_size * 4
         return {
-            "geometries": _view(d.geometries, GEOMETRY_DTYPE, d.triangle_count),
-            "attributes": _view(d.attributes, ATTRIBUTE_DTYPE, d.triangle_count),
-            "materials": _view(d.materials, MATERIAL_DTYPE, d.material_count),
-            "skybox": _view(d.skybox_rgba, np.float32, d.skybox_width * d.skybox_height * 4).reshape(
+            "geometries": _view(d.geometries, GEOMETRY_DTYPE, d.triangle_count, self),
+            "attributes": _view(d.attributes, ATTRIBUTE_DTYPE, d.triangle_count, self),
+            "materials": _view(d.materials, MATERIAL_DTYPE, d.material_count, self),
+            "srgb_atlases": _view(d.srgb_atlases, np.uint8, layer * d.srgb_atlas_count, self),
+            "linear_atlases": _view(d.linear_atlases, np.uint8, layer * d.linear_atlas_count, self),
+            "skybox": _view(d.skybox_rgba, np.float32, d.skybox_width * d.skybox_height * 4, self).reshape(
                 d.skybox_height, d.skybox_width, 4),
         }
+
+    def render_flags(self):
+        """The reference's `render` flags that the world's camera and sky luminance become
+        (headless.rs:14-64): they are not part of the .glb or the sky file."""
+        d = self.desc
+        flags = []
+        if not any(math.isnan(d.camera_position[k]) for k in range(3)):
+            flags += ["--camera-position", ",".join(f"{d.camera_position[k]:.9g}" for k in range(3))]
+        for name, v in (("--camera-yaw", d.camera_yaw_degrees), ("--camera-pitch", d.camera_pitch_degrees),
+                        ("--camera-fov", d.camera_fov_degrees)):
+            if not math.isnan(v):
+                flags += [name, f"{v:.9g}"]
+        if d.skybox_luminance >= 0:
+            flags += ["--skybox-luminance", f"{d.skybox_luminance:.9g}"]
+        return flags
+
+
+def export_glb(world_desc: WorldDesc, glb_path, sky_path=None):
+    """pt_world_export_glb: the world input as a .glb, and its sky as a FLOAT OpenEXR (or,
+    for a path ending in .hdr, exact RGBE). Returns the count of normal and tangent
+    directions that do not load back bit for bit (0 when the round trip is exact)."""
+    n = _u32(0)
+    _check(lib().pt_world_export_glb(C.byref(world_desc), os.fsencode(str(glb_path)),
+                                     None if sky_path is None else os.fsencode(str(sky_path)), C.byref(n)))
+    return n.value
+
+
+def save_scene(scene_desc: SceneDesc, path):
+    """pt_scene_save: the device-layout arrays (BVH included) as a .ptscene file."""
+    _check(lib().pt_scene_save(C.byref(scene_desc), os.fsencode(str(path))))
+
+
+class SceneFile:
+    """A loaded .ptscene file (pt_scene_file); `desc` borrows its arrays."""
+
+    def __init__(self, path):
+        h = C.c_void_p()
+        _check(lib().pt_scene_file_load(os.fsencode(str(path)), C.byref(h)))
+        self.handle = h
+        self.desc = SceneDesc()
+        _check(lib().pt_scene_file_desc(h, C.byref(self.desc)))
+
+    def close(self):
+        if self.handle:
+            lib().pt_scene_file_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class World:
@@ -426,15 +488,15 @@ class World:
         d = self.desc
         layer = d.atlas_size * d.atlas_size * 4
         return {
-            "geometries": _view(d.geometries, GEOMETRY_DTYPE, d.triangle_count),
-            "attributes": _view(d.attributes, ATTRIBUTE_DTYPE, d.triangle_count),
-            "materials": _view(d.materials, MATERIAL_DTYPE, d.material_count),
-            "bvh_nodes": _view(d.bvh_nodes, BVH_DTYPE, d.bvh_node_count),
-            "srgb_atlases": _view(d.srgb_atlases, np.uint8, layer * d.srgb_atlas_count),
-            "linear_atlases": _view(d.linear_atlases, np.uint8, layer * d.linear_atlas_count),
-            "skybox": _view(d.skybox_rgba, np.float32, d.skybox.width * d.skybox.height * 4),
-            "skybox_alias": _view(d.skybox_alias, SKY_ALIAS_DTYPE, d.skybox_alias_count),
-            "scene_alias": _view(d.scene_alias, SCENE_ALIAS_DTYPE, d.scene_alias_count),
+            "geometries": _view(d.geometries, GEOMETRY_DTYPE, d.triangle_count, self),
+            "attributes": _view(d.attributes, ATTRIBUTE_DTYPE, d.triangle_count, self),
+            "materials": _view(d.materials, MATERIAL_DTYPE, d.material_count, self),
+            "bvh_nodes": _view(d.bvh_nodes, BVH_DTYPE, d.bvh_node_count, self),
+            "srgb_atlases": _view(d.srgb_atlases, np.uint8, layer * d.srgb_atlas_count, self),
+            "linear_atlases": _view(d.linear_atlases, np.uint8, layer * d.linear_atlas_count, self),
+            "skybox": _view(d.skybox_rgba, np.float32, d.skybox.width * d.skybox.height * 4, self),
+            "skybox_alias": _view(d.skybox_alias, SKY_ALIAS_DTYPE, d.skybox_alias_count, self),
+            "scene_alias": _view(d.scene_alias, SCENE_ALIAS_DTYPE, d.scene_alias_count, self),
         }
 
 
@@ -574,12 +636,12 @@ class Assets:
     def arrays(self):
         d = self.world_desc()
         layer = d.atlas_size * d.atlas_size * 4
-        return {"geometries": _view(d.geometries, GEOMETRY_DTYPE, d.triangle_count),
-                "attributes": _view(d.attributes, ATTRIBUTE_DTYPE, d.triangle_count),
-                "materials": _view(d.materials, MATERIAL_DTYPE, d.material_count),
-                "srgb_atlases": _view(d.srgb_atlases, np.uint8, layer * d.srgb_atlas_count),
-                "linear_atlases": _view(d.linear_atlases, np.uint8, layer * d.linear_atlas_count),
-                "skybox": _view(d.skybox_rgba, np.float32, d.skybox_width * d.skybox_height * 4).reshape(
+        return {"geometries": _view(d.geometries, GEOMETRY_DTYPE, d.triangle_count, self),
+                "attributes": _view(d.attributes, ATTRIBUTE_DTYPE, d.triangle_count, self),
+                "materials": _view(d.materials, MATERIAL_DTYPE, d.material_count, self),
+                "srgb_atlases": _view(d.srgb_atlases, np.uint8, layer * d.srgb_atlas_count, self),
+                "linear_atlases": _view(d.linear_atlases, np.uint8, layer * d.linear_atlas_count, self),
+                "skybox": _view(d.skybox_rgba, np.float32, d.skybox_width * d.skybox_height * 4, self).reshape(
                     d.skybox_height, d.skybox_width, 4)}
 
     def close(self):

@@ -1,0 +1,478 @@
+"""Scene export (pt_assets.h): pt_world_export_glb and the .ptscene file. CPU only.
+
+- Round trip: loading an exported world (pt_assets_load_scene + the sky
+  decoder) gives the world's arrays back byte for byte: every triangle,
+  attribute, material, atlas texel and sky value of the generated Cornell box,
+  the 262k-triangle field scene, a sized scene and the 4.2M-triangle textured
+  scene (sky as exact RGBE .hdr), and of worlds loaded from glTF files.
+- The written files are also read by decoders written here, independent of
+  the library's own loaders, following the published formats: the GLB
+  container and glTF accessors (glTF 2.0), every PNG image with all five
+  filter types (PNG spec section 9), the uncompressed OpenEXR scanlines and
+  the run-length coded Radiance RGBE scanlines. Normals and tangents are
+  checked by a float32 restatement of the loader's normalisation
+  (1 / sqrt((x*x + y*y) + z*z), then a product per component).
+- The camera and the sky luminance are the reference's render flags
+  (headless.rs:14-64): the flags printed for a world rebuild its camera.
+- .ptscene: save, then load, gives every array and uniform of a built scene.
+"""
+import json
+import math
+import os
+import struct
+import subprocess
+import zlib
+
+import numpy as np
+import pytest
+
+import ptcore
+from glb_util import GlbBuilder, png
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PT_RENDER = os.path.join(ROOT, "path-tracer_amd", "pt-render")
+REF_GLB = "/root/reference/assets/glb"
+ATLAS = 8192
+
+
+# ------------------------------------------------------------ independent readers
+def read_glb(data):
+    magic, version, total = struct.unpack_from("<III", data, 0)
+    assert magic == 0x46546C67 and version == 2 and total == len(data)
+    jlen, jtype = struct.unpack_from("<II", data, 12)
+    assert jtype == 0x4E4F534A and jlen % 4 == 0
+    doc = json.loads(data[20:20 + jlen])
+    blen, btype = struct.unpack_from("<II", data, 20 + jlen)
+    assert btype == 0x004E4942 and 28 + jlen + blen == total
+    return doc, data[28 + jlen:28 + jlen + blen]
+
+
+def accessor(doc, bin_, i):
+    a = doc["accessors"][i]
+    v = doc["bufferViews"][a["bufferView"]]
+    assert a["componentType"] == 5126 and v["buffer"] == 0
+    n = {"VEC2": 2, "VEC3": 3, "VEC4": 4}[a["type"]]
+    off = v.get("byteOffset", 0) + a.get("byteOffset", 0)
+    assert off % 4 == 0 and v["byteLength"] >= a["count"] * n * 4
+    return np.frombuffer(bin_, np.float32, a["count"] * n, off).reshape(-1, n)
+
+
+def decode_png(data):
+    """8-bit, non-interlaced PNG (colour types 2 and 6) to an HxWxC uint8 array."""
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, ihdr = 8, b"", None
+    while pos < len(data):
+        n, kind = struct.unpack_from(">I4s", data, pos)
+        body = data[pos + 8:pos + 8 + n]
+        assert zlib.crc32(kind + body) == struct.unpack_from(">I", data, pos + 8 + n)[0]
+        if kind == b"IHDR":
+            ihdr = struct.unpack(">IIBBBBB", body)
+        elif kind == b"IDAT":
+            idat += body
+        pos += 12 + n
+    w, h, depth, ctype, _, _, interlace = ihdr
+    assert depth == 8 and ctype in (2, 6) and interlace == 0
+    bpp = 3 if ctype == 2 else 4
+    raw = zlib.decompress(idat)
+    stride = w * bpp
+    assert len(raw) == h * (stride + 1)
+    out = np.zeros((h, stride), np.int32)
+    prev = np.zeros(stride, np.int32)
+    for y in range(h):
+        f = raw[y * (stride + 1)]
+        line = np.frombuffer(raw, np.uint8, stride, y * (stride + 1) + 1).astype(np.int32)
+        if f == 0:
+            cur = line
+        elif f == 2:
+            cur = (line + prev) & 255
+        elif f == 1:  # Sub: a running sum per byte lane
+            cur = np.cumsum(line.reshape(w, bpp), 0).reshape(-1) & 255
+        else:  # Average and Paeth depend on the decoded left neighbour
+            cur = line.copy()
+            for x in range(stride):
+                a = int(cur[x - bpp]) if x >= bpp else 0
+                b = int(prev[x])
+                if f == 3:
+                    cur[x] = (cur[x] + (a + b) // 2) & 255
+                else:
+                    assert f == 4
+                    c = int(prev[x - bpp]) if x >= bpp else 0
+                    p = a + b - c
+                    pa, pb, pc = abs(p - a), abs(p - b), abs(p - c)
+                    pred = a if pa <= pb and pa <= pc else (b if pb <= pc else c)
+                    cur[x] = (cur[x] + pred) & 255
+        out[y] = cur
+        prev = cur
+    return out.astype(np.uint8).reshape(h, w, bpp)
+
+
+def decode_exr_float(data):
+    """Single-part scanline OpenEXR, NO_COMPRESSION, FLOAT channels, to HxWx4 (R, G, B, A)."""
+    assert struct.unpack_from("<II", data, 0) == (20000630, 2)
+    pos, attrs = 8, {}
+    while data[pos] != 0:
+        name_end = data.index(b"\0", pos)
+        type_end = data.index(b"\0", name_end + 1)
+        size = struct.unpack_from("<i", data, type_end + 1)[0]
+        attrs[data[pos:name_end].decode()] = data[type_end + 5:type_end + 5 + size]
+        pos = type_end + 5 + size
+    pos += 1
+    assert attrs["compression"] == b"\0" and attrs["lineOrder"] == b"\0"
+    x0, y0, x1, y1 = struct.unpack("<iiii", attrs["dataWindow"])
+    w, h = x1 - x0 + 1, y1 - y0 + 1
+    names, cl = [], attrs["channels"]
+    while cl[0] != 0:
+        e = cl.index(b"\0")
+        names.append(cl[:e].decode())
+        assert struct.unpack_from("<i", cl, e + 1)[0] == 2  # FLOAT
+        cl = cl[e + 17:]
+    offsets = struct.unpack_from("<%dQ" % h, data, pos)
+    img = np.zeros((h, w, 4), np.float32)
+    for row, off in enumerate(offsets):
+        y, size = struct.unpack_from("<ii", data, off)
+        assert y == y0 + row and size == 4 * w * len(names)
+        vals = np.frombuffer(data, np.float32, w * len(names), off + 8).reshape(len(names), w)
+        for k, n in enumerate(names):
+            img[row, :, "RGBA".index(n)] = vals[k]
+    return img
+
+
+def decode_hdr(data):
+    """Radiance RGBE (new-style run-length scanlines or flat) to HxWx3 float32: c * 2^(e - 136)."""
+    head_end = data.index(b"\n\n")
+    assert data.startswith(b"#?") and b"FORMAT=32-bit_rle_rgbe" in data[:head_end]
+    res_end = data.index(b"\n", head_end + 2)
+    ya, h, xa, w = data[head_end + 2:res_end].split()
+    assert ya == b"-Y" and xa == b"+X"
+    h, w = int(h), int(w)
+    pos, rgbe = res_end + 1, np.zeros((h, w, 4), np.uint8)
+    for y in range(h):
+        if data[pos] == 2 and data[pos + 1] == 2 and (data[pos + 2] << 8 | data[pos + 3]) == w:
+            pos += 4
+            for c in range(4):
+                x = 0
+                while x < w:
+                    n = data[pos]
+                    if n > 128:
+                        rgbe[y, x:x + n - 128, c] = data[pos + 1]
+                        x, pos = x + n - 128, pos + 2
+                    else:
+                        assert n > 0
+                        rgbe[y, x:x + n, c] = np.frombuffer(data, np.uint8, n, pos + 1)
+                        x, pos = x + n, pos + 1 + n
+                assert x == w
+        else:
+            rgbe[y] = np.frombuffer(data, np.uint8, 4 * w, pos).reshape(w, 4)
+            pos += 4 * w
+    assert pos == len(data)
+    e = rgbe[..., 3:4].astype(np.int32)
+    return np.where(e > 0, np.ldexp(rgbe[..., :3].astype(np.float32), e - 136), np.float32(0)).astype(np.float32)
+
+
+def load_unit(v):
+    """The loader's normalisation of a stored direction through an identity node transform."""
+    v = np.asarray(v, np.float32) + np.float32(0)
+    s = (v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2]
+    inv = np.float32(1) / np.sqrt(s)
+    return v * inv[:, None]
+
+
+def same_bytes(a, b):
+    return a.shape == b.shape and a.tobytes() == b.tobytes()
+
+
+# ------------------------------------------------------------ helpers
+def export(desc, tmp_path, name, sky_ext=".exr"):
+    glb, sky = tmp_path / (name + ".glb"), tmp_path / (name + "_sky" + sky_ext)
+    inexact = ptcore.export_glb(desc, glb, sky)
+    return glb, sky, inexact
+
+
+def assert_reload_identical(world_arrays, glb, sky):
+    a = ptcore.Assets(glb, sky)
+    got = a.arrays()
+    for k, v in world_arrays.items():
+        assert same_bytes(got[k], v), k
+    assert a.warnings() == []
+    return a
+
+
+GENERATED = {
+    "cornell": (ptcore.SCENE_CORNELL, 1, 256, 256, 0),
+    "field": (ptcore.SCENE_FIELD, 2, 1920, 1080, 0),
+    "sized": (ptcore.SCENE_SIZED, 5, 512, 512, 40000),
+}
+
+
+@pytest.mark.parametrize("name", list(GENERATED))
+def test_generated_world_reloads_byte_identical(tmp_path, name):
+    kind, seed, w, h, tris = GENERATED[name]
+    g = ptcore.Generated(kind, seed, w, h, tris)
+    glb, sky, inexact = export(g.desc, tmp_path, name)
+    assert inexact == 0
+    assert_reload_identical(g.arrays(), glb, sky)
+
+
+def test_glb_layout_read_independently(tmp_path):
+    g = ptcore.Generated(ptcore.SCENE_SIZED, 5, 512, 512, 20000)
+    glb, sky, _ = export(g.desc, tmp_path, "sized")
+    doc, bin_ = read_glb(glb.read_bytes())
+    arr = g.arrays()
+    geo, att, mats = arr["geometries"], arr["attributes"], arr["materials"]
+    assert doc["asset"]["version"] == "2.0"
+    assert doc["scenes"][doc["scene"]]["nodes"] == [0] and set(doc["nodes"][0]) == {"mesh"}
+    assert len(doc["materials"]) == len(mats)
+    t = 0
+    for prim in doc["meshes"][0]["primitives"]:
+        assert prim["mode"] == 4 and "indices" not in prim
+        at = prim["attributes"]
+        p = accessor(doc, bin_, at["POSITION"])
+        n_tri = len(p) // 3
+        sl = slice(t, t + n_tri)
+        assert np.all(att["material"][sl] == prim["material"])
+        assert same_bytes(p.reshape(n_tri, 9), geo["v"][sl])
+        acc = doc["accessors"][at["POSITION"]]
+        assert np.array_equal(np.float32(acc["min"]), p.min(0)) and np.array_equal(np.float32(acc["max"]), p.max(0))
+        nrm = load_unit(accessor(doc, bin_, at["NORMAL"])).reshape(n_tri, 9)
+        assert same_bytes(nrm, att["n"][sl])
+        tan = accessor(doc, bin_, at["TANGENT"]).reshape(n_tri, 3, 4)
+        assert not np.any(np.all(tan == np.float32([1, 0, 0, 1]), -1))  # no loader placeholder
+        for c in range(3):
+            want = att["t%d" % c][sl]
+            assert same_bytes(load_unit(tan[:, c, :3]), want[:, :3])
+            assert same_bytes(np.ascontiguousarray(tan[:, c, 3]), np.ascontiguousarray(want[:, 3]))
+        assert same_bytes(accessor(doc, bin_, at["TEXCOORD_0"]).reshape(n_tri, 6), att["uv"][sl])
+        t += n_tri
+    assert t == len(geo)
+    for m, gm in zip(mats, doc["materials"]):
+        pbr = gm["pbrMetallicRoughness"]
+        assert np.array_equal(np.float32(pbr["baseColorFactor"]), m["base_color"])
+        assert np.float32(pbr["metallicFactor"]) == m["base_metalness"]
+        # glTF roughness is the diffuse roughness; specular roughness is an OpenPBR extra (world.rs:2046-2111)
+        assert np.float32(pbr["roughnessFactor"]) == m["base_diffuse_roughness"]
+        assert np.float32(gm["extras"]["OpenPBR_specular_roughness"]) == m["specular_roughness"]
+        assert np.float32(gm["extensions"]["KHR_materials_ior"]["ior"]) == m["specular_ior"]
+        lum = np.float32(gm["extensions"]["KHR_materials_emissive_strength"]["emissiveStrength"])
+        assert np.array_equal(np.float32(gm["emissiveFactor"]), m["emission_color_and_luminance"][:3])
+        assert lum == m["emission_color_and_luminance"][3]
+    # the sky, read here
+    assert same_bytes(decode_exr_float(sky.read_bytes()), arr["skybox"])
+
+
+def test_large_textured_scene_reloads_byte_identical(tmp_path):
+    g = ptcore.Generated(ptcore.SCENE_LARGE, 3, 3840, 2160)
+    d = g.desc
+    assert d.triangle_count > 4_000_000 and d.srgb_atlas_count + d.linear_atlas_count > 0
+    glb, sky, inexact = export(d, tmp_path, "large", ".hdr")
+    assert inexact == 0
+    arr = g.arrays()
+    a = assert_reload_identical(arr, glb, sky)
+    del a
+    # images: one RGBA PNG per 2048^2 texture; sky: exact RGBE
+    doc, bin_ = read_glb(glb.read_bytes())
+    assert len(doc["images"]) == 8
+    for im in doc["images"]:
+        v = doc["bufferViews"][im["bufferView"]]
+        data = bin_[v["byteOffset"]:v["byteOffset"] + v["byteLength"]]
+        assert im["mimeType"] == "image/png" and struct.unpack(">II", data[16:24]) == (2048, 2048)
+    sky_img = decode_hdr(sky.read_bytes())
+    assert same_bytes(sky_img, np.ascontiguousarray(arr["skybox"][..., :3]))
+    assert np.all(arr["skybox"][..., 3] == 1)
+
+
+def textured_glb():
+    """A quad with four PNG textures: RGB and RGBA into the sRGB atlas, grey+alpha and
+    RGBA into the linear one; contents chosen so every PNG filter type is taken."""
+    rng = np.random.default_rng(11)
+    ramp = (np.add.outer(np.arange(37), np.arange(29) * 3) % 256).astype(np.uint8)
+    rgb = np.stack([ramp, ramp[::-1], np.full_like(ramp, 77)], -1)
+    noise = rng.integers(0, 256, (23, 41, 4), dtype=np.uint8)
+    la = np.stack([np.tile(np.arange(16, dtype=np.uint8) * 16, (9, 1)), np.full((9, 16), 200, np.uint8)], -1)
+    smooth = (np.add.outer(np.arange(31) * 5, np.arange(33) * 7)[..., None] + np.arange(4) * 40) % 256
+    g = GlbBuilder()
+    t = [g.image(png(x)) for x in (rgb, noise, la, smooth.astype(np.uint8))]
+    p = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], np.float32)
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+    attrs = {"POSITION": g.accessor(p), "TEXCOORD_0": g.accessor(uv)}
+    g.doc["materials"] = [{"pbrMetallicRoughness": {"baseColorTexture": {"index": t[0]},
+                                                    "metallicRoughnessTexture": {"index": t[2]}},
+                           "emissiveTexture": {"index": t[1]}, "normalTexture": {"index": t[3], "scale": 0.75}}]
+    g.doc["meshes"] = [{"primitives": [{"attributes": attrs, "indices": g.accessor(np.array([0, 1, 2, 0, 2, 3], np.uint32)),
+                                        "material": 0}]}]
+    # an oblique rotation: no normal or tangent component is zero (the loader's -0 in a
+    # direction cannot be stored: loading turns it into +0, and the exporter counts it)
+    axis = np.array([1.0, 2.0, 3.0]) / math.sqrt(14.0)
+    q = [*(axis * math.sin(0.35)), math.cos(0.35)]
+    g.doc["nodes"] = [{"mesh": 0, "rotation": q, "translation": [0.25, -1.5, 3.0]}]
+    return g.build()
+
+
+def test_loaded_textured_world_round_trip_and_png_images(tmp_path):
+    src = tmp_path / "src.glb"
+    src.write_bytes(textured_glb())
+    gen_sky = ptcore.Generated(ptcore.SCENE_FIELD, 2, 64, 64)
+    sky0 = tmp_path / "src_sky.exr"
+    ptcore.export_glb(gen_sky.desc, tmp_path / "unused.glb", sky0)
+    a0 = ptcore.Assets(src, sky0)
+    arr = a0.arrays()
+    glb, sky, inexact = export(a0.world_desc(), tmp_path, "out")
+    assert inexact == 0
+    assert_reload_identical(arr, glb, sky)
+    # every image decoded here equals its atlas rectangle
+    doc, bin_ = read_glb(glb.read_bytes())
+    m, gm = arr["materials"][0], doc["materials"][0]
+    slots = [("base_color", gm["pbrMetallicRoughness"]["baseColorTexture"], "srgb_atlases"),
+             ("emission", gm["emissiveTexture"], "srgb_atlases"),
+             ("metallic_roughness", gm["pbrMetallicRoughness"]["metallicRoughnessTexture"], "linear_atlases"),
+             ("normal", gm["normalTexture"], "linear_atlases")]
+    assert gm["normalTexture"]["scale"] == 0.75
+    filters = set()
+    for field, ref, atlas in slots:
+        layer = m[field + "_tex_layer"]
+        x, y, w, h = (int(round(float(c) * ATLAS)) for c in m[field + "_uv"])
+        im = doc["images"][doc["textures"][ref["index"]]["source"]]
+        v = doc["bufferViews"][im["bufferView"]]
+        data = bin_[v["byteOffset"]:v["byteOffset"] + v["byteLength"]]
+        raw = zlib.decompress(b"".join(
+            data[p + 8:p + 8 + struct.unpack_from(">I", data, p)[0]] for p in _chunks(data, b"IDAT")))
+        filters |= {raw[r * (w * 4 + 1)] for r in range(h)}
+        img = decode_png(data)
+        want = arr[atlas].reshape(-1, ATLAS, ATLAS, 4)[layer, y:y + h, x:x + w]
+        assert img.shape == (h, w, 4) and np.array_equal(img, want), field
+    assert filters == {0, 1, 2, 3, 4}
+
+
+def _chunks(data, kind):
+    pos = 8
+    while pos < len(data):
+        n, k = struct.unpack_from(">I4s", data, pos)
+        if k == kind:
+            yield pos
+        pos += 12 + n
+
+
+@pytest.mark.skipif(not os.path.exists(REF_GLB), reason="the reference's asset files are not present")
+def test_reference_box_scene_round_trip(tmp_path):
+    sky0 = tmp_path / "sky.exr"
+    g = ptcore.Generated(ptcore.SCENE_CORNELL, 1, 64, 64)  # the desc borrows g's arrays
+    ptcore.export_glb(g.desc, tmp_path / "unused.glb", sky0)
+    a0 = ptcore.Assets(os.path.join(REF_GLB, "box.glb"), sky0)
+    glb, sky, inexact = export(a0.world_desc(), tmp_path, "box")
+    assert inexact == 0
+    assert_reload_identical(a0.arrays(), glb, sky)
+
+
+def test_hdr_sky_is_refused_unless_exact(tmp_path):
+    g = ptcore.Generated(ptcore.SCENE_FIELD, 2, 64, 64)  # a gradient sky, not RGBE-quantised
+    with pytest.raises(ptcore.PtError, match="RGBE"):
+        ptcore.export_glb(g.desc, tmp_path / "f.glb", tmp_path / "f.hdr")
+    # a world whose sky values are RGBE values: the .hdr decodes (here) to them
+    sky = np.zeros((3, 9, 4), np.float32)
+    sky[..., 3] = 1
+    sky[0, :, :3] = np.ldexp(np.float32([200, 17, 0]), 130 - 136)  # one exponent, a run
+    sky[1, :, :3] = np.ldexp(np.float32(np.arange(27).reshape(9, 3) + 128), -7)
+    d = ptcore.WorldDesc.from_buffer_copy(g.desc)
+    d.skybox_rgba, d.skybox_width, d.skybox_height = sky.ctypes.data, 9, 3
+    ptcore.export_glb(d, tmp_path / "s.glb", tmp_path / "s.hdr")
+    assert same_bytes(decode_hdr((tmp_path / "s.hdr").read_bytes()), np.ascontiguousarray(sky[..., :3]))
+    assert same_bytes(ptcore.Assets(tmp_path / "s.glb", tmp_path / "s.hdr").arrays()["skybox"], sky)
+
+
+def with_materials(desc, mats, attributes=None):
+    d = ptcore.WorldDesc.from_buffer_copy(desc)
+    d.materials, d.material_count = mats.ctypes.data, len(mats)
+    if attributes is not None:
+        d.attributes = attributes.ctypes.data
+    return d
+
+
+def test_materials_out_of_first_use_order_load_renumbered(tmp_path):
+    g = ptcore.Generated(ptcore.SCENE_CORNELL, 1, 64, 64)
+    arr = g.arrays()
+    perm = np.arange(len(arr["materials"]))[::-1].copy()  # world material i -> perm[i]
+    mats = np.zeros_like(arr["materials"])
+    mats[perm] = arr["materials"]
+    att = arr["attributes"].copy()
+    att["material"] = perm[att["material"]]
+    d = with_materials(g.desc, mats, att)
+    glb, sky, _ = export(d, tmp_path, "perm")
+    got = ptcore.Assets(glb, sky).arrays()
+    assert same_bytes(got["materials"], arr["materials"]) and same_bytes(got["attributes"], arr["attributes"])
+
+
+def test_what_the_format_cannot_hold_is_refused(tmp_path):
+    g = ptcore.Generated(ptcore.SCENE_CORNELL, 1, 64, 64)
+    base = g.arrays()["materials"]
+    for field, value, msg in [("specular_haze", 1.5, "OpenPBR"), ("base_diffuse_roughness", -0.25, "roughnessFactor"),
+                              ("base_color", [0.5, 1.25, 0.5, 1.0], "baseColorFactor"), ("specular_ior", 0.5, "ior"),
+                              ("normal_scale", 0.5, "normal")]:
+        mats = base.copy()
+        mats[0][field] = value
+        with pytest.raises(ptcore.PtError, match=msg) as e:
+            ptcore.export_glb(with_materials(g.desc, mats), tmp_path / "bad.glb")
+        assert e.value.code == ptcore.PT_ERR_INVALID
+    with pytest.raises(ptcore.PtError):
+        ptcore.export_glb(g.desc, tmp_path / "no_such_dir" / "x.glb")
+
+
+def test_render_flags_rebuild_the_camera(tmp_path):
+    for kind, seed, w, h in [(ptcore.SCENE_CORNELL, 1, 256, 256), (ptcore.SCENE_FIELD, 2, 640, 360)]:
+        g = ptcore.Generated(kind, seed, w, h)
+        flags = g.render_flags()
+        glb, sky, _ = export(g.desc, tmp_path, "cam")
+        opts = dict(zip(flags[::2], flags[1::2]))
+        a = ptcore.Assets(glb, sky)
+        w1 = a.world(skybox_luminance=float(opts["--skybox-luminance"]),
+                     camera_position=[float(c) for c in opts["--camera-position"].split(",")],
+                     yaw=float(opts["--camera-yaw"]), pitch=float(opts["--camera-pitch"]),
+                     fov=float(opts["--camera-fov"]), aspect=w / h)
+        w0 = ptcore.World(g.desc, keep=g)
+        assert bytes(w1.desc.camera) == bytes(w0.desc.camera)
+        assert bytes(w1.desc.skybox) == bytes(w0.desc.skybox)
+        assert bytes(w1.desc.scene_uniforms) == bytes(w0.desc.scene_uniforms)
+        for k in ("bvh_nodes", "scene_alias", "skybox_alias"):
+            assert same_bytes(w1.arrays()[k], w0.arrays()[k]), k
+
+
+def scene_fields(desc):
+    return {"camera": bytes(desc.camera), "skybox": bytes(desc.skybox), "scene_uniforms": bytes(desc.scene_uniforms),
+            "atlas_size": desc.atlas_size}
+
+
+@pytest.mark.parametrize("name", ["cornell", "sized"])
+def test_ptscene_round_trip(tmp_path, name):
+    kind, seed, w, h, tris = GENERATED[name]
+    world = ptcore.World.generate(kind, seed, w, h, tris)
+    path = tmp_path / (name + ".ptscene")
+    ptcore.save_scene(world.desc, path)
+    f = ptcore.SceneFile(path)
+    got, want = ptcore.World.arrays(f), world.arrays()
+    for k, v in want.items():
+        assert same_bytes(got[k], v), k
+    assert scene_fields(f.desc) == scene_fields(world.desc)
+    f.close()
+    # damaged files are refused with a message, not read past their end
+    data = path.read_bytes()
+    for bad in (data[:len(data) // 2], b"PTSCENE0" + data[8:], data[:-16]):
+        path.write_bytes(bad)
+        with pytest.raises(ptcore.PtError):
+            ptcore.SceneFile(path)
+
+
+@pytest.mark.skipif(not os.path.exists(PT_RENDER), reason="pt-render is not built")
+def test_pt_render_export_and_ptscene_flags(tmp_path):
+    glb, sky, ps = tmp_path / "c.glb", tmp_path / "c_sky.exr", tmp_path / "c.ptscene"
+    r = subprocess.run([PT_RENDER, "--scene", "cornell", "--width", "256", "--height", "256", "--export-glb", str(glb),
+                        "--export-sky", str(sky), "--export-ptscene", str(ps)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    g = ptcore.Generated(ptcore.SCENE_CORNELL, 1, 256, 256)
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith("render flags:")][0]
+    assert line.split()[2:] == g.render_flags() + ["--width", "256", "--height", "256"]
+    assert_reload_identical(g.arrays(), glb, sky)
+    world = ptcore.World(g.desc, keep=g)
+    f = ptcore.SceneFile(ps)
+    assert scene_fields(f.desc) == scene_fields(world.desc)
+    assert same_bytes(ptcore.World.arrays(f)["bvh_nodes"], world.arrays()["bvh_nodes"])
+    r = subprocess.run([PT_RENDER, "--scene", str(ps), "--camera-fov", "30"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "camera" in r.stderr
