@@ -129,3 +129,41 @@ def test_pt_render_writes_the_image(tmp_path):
         assert decoded["png"].shape == (64, 96, 3)
         assert np.array_equal(decoded["png"], decoded["webp"])
         assert np.abs(decoded["jpg"] - decoded["png"]).mean() < 4.0
+
+
+def render_tile(ctx, scene_desc, samples=4):
+    s = ptcore.Scene(ctx, scene_desc)
+    t = ptcore.Tracer(ctx, s, 64, 64)
+    t.set_tile((1920, 1080), (800, 500), (64, 64))
+    t.set_max_bounces(6)
+    t.trace(samples)
+    out = t.accumulation().copy()
+    t.close()
+    s.close()
+    return out
+
+
+def test_exported_field_scene_and_its_ptscene_render_as_the_original(ctx, field, tmp_path):
+    """The 262k-triangle field scene written as .glb + FLOAT EXR sky (loaded back with
+    its render flags), and its built arrays as .ptscene, render bit for bit as the world
+    they came from."""
+    want = render_tile(ctx, field.desc)
+    assert np.isfinite(want).all() and want[..., :3].sum() > 0
+    ptcore.save_scene(field.desc, tmp_path / "field.ptscene")
+    f = ptcore.SceneFile(tmp_path / "field.ptscene")
+    got = render_tile(ctx, f.desc)
+    print("ptscene: bit-exact", np.array_equal(got, want), flush=True)
+    assert np.array_equal(got, want)
+    f.close()
+    g = ptcore.Generated(ptcore.SCENE_FIELD, 2, 1920, 1080)
+    assert ptcore.export_glb(g.desc, tmp_path / "field.glb", tmp_path / "field_sky.exr") == 0
+    flags = g.render_flags()
+    opts = dict(zip(flags[::2], flags[1::2]))
+    a = ptcore.Assets(tmp_path / "field.glb", tmp_path / "field_sky.exr")
+    w = a.world(skybox_luminance=float(opts["--skybox-luminance"]),
+                camera_position=[float(c) for c in opts["--camera-position"].split(",")],
+                yaw=float(opts["--camera-yaw"]), pitch=float(opts["--camera-pitch"]),
+                fov=float(opts["--camera-fov"]), aspect=1920 / 1080)
+    got = render_tile(ctx, w.desc)
+    print("glb: bit-exact", np.array_equal(got, want), flush=True)
+    assert np.array_equal(got, want)
