@@ -102,7 +102,10 @@ pt_status pt_world_export_glb(const pt_world_desc* desc, const char* glb_path, c
 /* The device-layout arrays of a scene (pt_scene_desc, BVH included) as a
  * .ptscene file, and back. No counterpart in the reference: it lets a tool
  * hand over the identical serialized BVH. A loaded file owns its arrays;
- * pt_scene_file_desc borrows them until pt_scene_file_destroy. */
+ * pt_scene_file_desc borrows them until pt_scene_file_destroy. Loading
+ * refuses (PT_ERR_INVALID) a damaged file and one whose arrays
+ * pt_scene_create would refuse (an index the kernels follow out of range, a
+ * BVH that is not a tree), so it can be checked without a device. */
 typedef struct pt_scene_file pt_scene_file;
 pt_status pt_scene_save(const pt_scene_desc* desc, const char* path);
 pt_status pt_scene_file_load(const char* path, pt_scene_file** out);

@@ -119,9 +119,13 @@ pt_status upload(T** dst, const T* src, size_t count, size_t& footprint, const c
     return PT_OK;
 }
 
+}  // namespace
+
+namespace ptcore {
+
 // BVH validity and depth: every child pair, leaf range and node is in range,
 // and the tree is acyclic (depth bounded by the node count).
-pt_status validate_bvh(const pt_scene_desc* d, uint32_t& max_depth) {
+static pt_status validate_bvh(const pt_scene_desc* d, uint32_t& max_depth) {
     if (d->bvh_node_count == 0 || !d->bvh_nodes) return set_error(PT_ERR_INVALID, "scene has no BVH nodes");
     max_depth = 0;
     std::vector<std::pair<uint32_t, uint32_t>> stack;
@@ -197,7 +201,7 @@ pt_status validate_scene(const pt_scene_desc* d, uint32_t& depth) {
     return validate_bvh(d, depth);
 }
 
-}  // namespace
+}  // namespace ptcore
 
 extern "C" {
 

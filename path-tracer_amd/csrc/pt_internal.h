@@ -15,6 +15,9 @@ namespace ptcore {
 pt_status set_error(pt_status code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 // `s` with every byte that is not part of well-formed UTF-8 text replaced by '?'.
 std::string utf8_clean(const char* s);
+// The checks pt_scene_create makes before upload: every index the kernels
+// follow is in range and the BVH is a tree (its depth is returned).
+pt_status validate_scene(const pt_scene_desc* d, uint32_t& depth);
 
 // Host-side prepared world (the arrays World::take_updates hands to the GPU).
 struct World {

@@ -513,21 +513,14 @@ pt_status pt_scene_file_load(const char* path, pt_scene_file** out) {
             if (std::fread(hdr, 4, 2, f) != 2 || std::fread(&n, 8, 1, f) != 1) return bad("truncated section header");
             const long at = std::ftell(f);
             if (at < 0 || n > (uint64_t)(file_size - at)) return bad("section larger than the file");
-            std::vector<uint8_t> b(n);
-            if (n && std::fread(b.data(), 1, n, f) != n) return bad("truncated section");
-            std::fseek(f, (long)((16 - n % 16) % 16), SEEK_CUR);
+            // each section is read straight into its array (no second copy of large ones)
             auto take = [&](auto& vec) {
                 using T = typename std::decay_t<decltype(vec)>::value_type;
-                if (n % sizeof(T)) return false;
+                if (n % sizeof(T) || n / sizeof(T) > 0xffffffffull) return false;
                 vec.resize(n / sizeof(T));
-                if (n) std::memcpy(vec.data(), b.data(), n);
-                return true;
+                return n == 0 || std::fread(vec.data(), 1, n, f) == n;
             };
-            auto take_one = [&](void* dst, size_t size) {
-                if (n != size) return false;
-                std::memcpy(dst, b.data(), size);
-                return true;
-            };
+            auto take_one = [&](void* dst, size_t size) { return n == size && std::fread(dst, 1, size, f) == size; };
             bool ok = true;
             switch (hdr[0]) {
                 case tag("GEOM"): ok = take(s->geo); break;
@@ -543,9 +536,10 @@ pt_status pt_scene_file_load(const char* path, pt_scene_file** out) {
                 case tag("SKYA"): ok = take(s->sky_alias); break;
                 case tag("SCNU"): ok = take_one(&s->uniforms, sizeof(PtSceneUniforms)); break;
                 case tag("SCNA"): ok = take(s->scene_alias); break;
-                default: break;  // unknown sections are skipped
+                default: ok = std::fseek(f, (long)n, SEEK_CUR) == 0; break;  // unknown sections are skipped
             }
-            if (!ok) return bad("section size does not fit its records");
+            if (!ok) return bad("section size does not fit its records, or the section is truncated");
+            if (std::fseek(f, (long)((16 - n % 16) % 16), SEEK_CUR) != 0) return bad("truncated section padding");
         }
     } catch (const std::bad_alloc&) {
         std::fclose(f);
@@ -561,6 +555,17 @@ pt_status pt_scene_file_load(const char* path, pt_scene_file** out) {
     s->linear_count = layer ? (uint32_t)(s->linear.size() / layer) : 0;
     if (s->sky.size() != (size_t)s->skybox.width * s->skybox.height * 4)
         return delete s, set_error(PT_ERR_INVALID, "%s: sky size differs from its uniforms", path);
+    if (s->srgb_count > 0x7fffffffu || s->linear_count > 0x7fffffffu)
+        return delete s, set_error(PT_ERR_INVALID, "%s: too many atlas layers", path);
+    // the indices the kernels follow, checked as pt_scene_create checks them
+    pt_scene_desc d;
+    uint32_t depth = 0;
+    pt_scene_file_desc(s, &d);
+    if (pt_status st = validate_scene(&d, depth)) {
+        const std::string why = pt_last_error();
+        delete s;
+        return set_error(st, "%s: %s", path, why.c_str());
+    }
     *out = s;
     return PT_OK;
 }
