@@ -118,7 +118,9 @@ def load_traffic(name):
 def wave_knobs(args):
     """The wavefront configuration fields set on the command line."""
     return {k: v for k, v in (("traversal", args.traversal), ("sort_key", args.sort), ("lds_stack_depth", args.lds),
-                              ("streams", args.streams), ("chunk_paths", args.chunk))
+                              ("streams", args.streams), ("chunk_paths", args.chunk),
+                              ("trace_waves_per_cu", args.trace_waves), ("shade_window", args.shade_window),
+                              ("shade_blocks_per_cu", args.shade_blocks))
             if v is not None}
 
 
@@ -616,6 +618,10 @@ def main():
     ap.add_argument("--chunk", type=int, default=None, help="paths per wavefront chunk (default: library default)")
     ap.add_argument("--streams", type=int, default=None, choices=[0, 1, 2],
                     help="wavefront chunk lanes on separate streams (default: library default)")
+    ap.add_argument("--trace-waves", type=int, default=None, help="traversal grid: one-wave blocks per CU")
+    ap.add_argument("--shade-window", type=int, default=None, choices=[1, 2, 4],
+                    help="queue rounds a shade block reorders into hits and misses")
+    ap.add_argument("--shade-blocks", type=int, default=None, help="shade grid: 256-thread blocks per CU")
     ap.add_argument("--trace-union", nargs=2, metavar=("TRACE_DIR", "BENCH_JSON"),
                     help="no GPU: the traversal busy time of a rocprofv3 kernel trace of this bench")
     args = ap.parse_args()
