@@ -106,6 +106,15 @@ def cpu_baseline(world, w, h, bounces, samples, repeats, seconds_per_run=4.0):
                       f"{((st['closest_rays'] + st['shadow_rays']) / dt):.4g}"}
 
 
+def load_l1(name):
+    """The L1 data-return busy fraction of the traversal from the committed rocprofv3 --pmc summary, if present."""
+    p = os.path.join(ROOT, "profiles", f"extend_l1_{name}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("td_busy")
+
+
 def load_traffic(name):
     """HBM bytes per traversal launch from the committed rocprofv3 --pmc summary, if present."""
     p = os.path.join(ROOT, "profiles", f"extend_traffic_{name}.json")
@@ -239,6 +248,18 @@ def measure_job(ctx, name, steps, warmup, schedule_name="wavefront"):
                             "avg_launch_ms": one_s * 1e3, "launches_per_step": one_per_step,
                             "traffic_frac": (traffic / one_s / 1e9 / HBM_PEAK_GBS) if traffic else None,
                             "note": "the same job on one lane in separate steps: no launch overlaps another kernel"}
+        # the traversal's nearest ceiling: the L1's rate of divergent 16-byte lane accesses
+        info = ctx.info()
+        accesses = (4 * (counts["pair_tests"] + counts["shadow_pair_tests"]) +
+                    3 * (counts["tri_tests"] + counts["shadow_tri_tests"]))
+        cu_cycles = one_s * one_per_step * info["clock_khz"] * 1e3 * info["cu_count"]
+        roof["one_lane"]["l1_gather"] = {
+            "lane_accesses_per_cu_cycle": accesses / cu_cycles if cu_cycles else None,
+            "td_busy": load_l1(name),
+            "cu_count": info["cu_count"], "clock_mhz": info["clock_khz"] / 1e3,
+            "note": "lane accesses = 4 per child-pair test + 3 per triangle test (loads of at most 16 B), over "
+                    "the one-lane traversal time at the peak clock; td_busy is the L1 data-return unit's busy "
+                    "fraction from the committed rocprofv3 counters"}
         out["roofline"] = roof
         out["kernel_ms_per_step"] = {"extend_busy": busy_s_per_step * 1e3,
                                      "extend_events": ev_all["extend_ms"] / extra_steps,

@@ -60,6 +60,17 @@ pt_status pt_context_create(int device, pt_context** out);
 pt_status pt_context_destroy(pt_context* ctx);
 /* Wait for all work queued on the context's stream. */
 pt_status pt_context_synchronize(pt_context* ctx);
+/* The device facts behind launch sizes and rate reports. No counterpart in
+ * the reference (wgpu's adapter limits play this part there). */
+typedef struct pt_context_info {
+    int32_t device;
+    uint32_t cu_count;      /* compute units */
+    uint32_t clock_khz;     /* peak engine clock */
+    uint32_t l2_bytes;      /* L2 cache of one XCD */
+    uint64_t hbm_bytes;     /* device memory */
+    char arch[64];          /* e.g. "gfx950:sramecc+:xnack-" */
+} pt_context_info;
+pt_status pt_context_get_info(const pt_context* ctx, pt_context_info* info);
 
 /* -------------------------------------------------------------- host world */
 /* Input of the host scene preparation: a triangle soup in input order (the

@@ -226,6 +226,9 @@ pt_status pt_context_create(int device, pt_context** out) {
         return hip_error(e, "hipGetDeviceProperties");
     }
     c->cu_count = (uint32_t)prop.multiProcessorCount;
+    c->clock_khz = (uint32_t)prop.clockRate;
+    c->l2_bytes = (uint32_t)prop.l2CacheSize;
+    c->hbm_bytes = (uint64_t)prop.totalGlobalMem;
     std::snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -259,6 +262,18 @@ pt_status pt_context_destroy(pt_context* c) {
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return e == hipSuccess ? PT_OK : hip_error(e, "work pending at context destruction");
+}
+
+pt_status pt_context_get_info(const pt_context* c, pt_context_info* info) {
+    if (!c || !info) return set_error(PT_ERR_INVALID, "pt_context_get_info: null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->device = c->device;
+    info->cu_count = c->cu_count;
+    info->clock_khz = c->clock_khz;
+    info->l2_bytes = c->l2_bytes;
+    info->hbm_bytes = c->hbm_bytes;
+    std::snprintf(info->arch, sizeof(info->arch), "%s", c->arch);
+    return PT_OK;
 }
 
 pt_status pt_context_synchronize(pt_context* c) {

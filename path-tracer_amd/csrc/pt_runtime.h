@@ -22,6 +22,8 @@ inline pt_status use_device(int device) {
 struct Context {
     int device = 0;
     uint32_t cu_count = 0;
+    uint32_t clock_khz = 0, l2_bytes = 0;
+    uint64_t hbm_bytes = 0;
     char arch[64] = {0};
     hipStream_t stream = nullptr;
     float* srgb_lut = nullptr;            // 256-entry sRGB decode table

@@ -90,6 +90,11 @@ assert BVH_DTYPE.itemsize == 32 and SCENE_ALIAS_DTYPE.itemsize == 16 and SKY_ALI
 _p = C.c_void_p
 
 
+class ContextInfo(C.Structure):
+    _fields_ = [("device", C.c_int32), ("cu_count", C.c_uint32), ("clock_khz", C.c_uint32), ("l2_bytes", C.c_uint32),
+                ("hbm_bytes", C.c_uint64), ("arch", C.c_char * 64)]
+
+
 class SceneDesc(C.Structure):
     """pt_scene_desc: device-layout arrays of one scene (bind group 1)."""
     _fields_ = [
@@ -192,6 +197,7 @@ _SIGNATURES = [
     ("pt_context_create", _i32, [C.c_int, C.POINTER(_p)]),
     ("pt_context_destroy", _i32, [_p]),
     ("pt_context_synchronize", _i32, [_p]),
+    ("pt_context_get_info", _i32, [_p, C.POINTER(ContextInfo)]),
     ("pt_world_build", _i32, [C.POINTER(WorldDesc), C.POINTER(_p)]),
     ("pt_world_destroy", _i32, [_p]),
     ("pt_world_scene_desc", _i32, [_p, C.POINTER(SceneDesc)]),
@@ -740,6 +746,13 @@ class Context:
 
     def synchronize(self):
         _check(lib().pt_context_synchronize(self.handle))
+
+    def info(self):
+        """pt_context_get_info: CU count, peak clock, L2 and device memory of the context's GPU."""
+        i = ContextInfo()
+        _check(lib().pt_context_get_info(self.handle, C.byref(i)))
+        return {"device": i.device, "cu_count": i.cu_count, "clock_khz": i.clock_khz, "l2_bytes": i.l2_bytes,
+                "hbm_bytes": i.hbm_bytes, "arch": i.arch.decode("utf-8", "replace")}
 
     def close(self):
         if self.handle:

@@ -305,3 +305,11 @@ def test_random_openpbr_materials_match_oracle(ctx, seed):
         assert_parity(t.accumulation(), ref, f"materials seed {seed} schedule={schedule}")
         t.close()
     scene.close()
+
+
+def test_context_info(ctx):
+    """pt_context_get_info reports the device the kernels were built for."""
+    i = ctx.info()
+    print(i, flush=True)
+    assert i["arch"].startswith("gfx950") and i["cu_count"] > 0 and i["clock_khz"] > 0
+    assert i["hbm_bytes"] > (1 << 30) and i["l2_bytes"] > 0
