@@ -60,8 +60,8 @@ pt_status pt_context_create(int device, pt_context** out);
 pt_status pt_context_destroy(pt_context* ctx);
 /* Wait for all work queued on the context's stream. */
 pt_status pt_context_synchronize(pt_context* ctx);
-/* The device facts behind launch sizes and rate reports. No counterpart in
- * the reference (wgpu's adapter limits play this part there). */
+/* The device facts behind launch sizes and rate reports; replaces the
+ * adapter.limits() / adapter.get_info() queries of gpu.rs:39-44. */
 typedef struct pt_context_info {
     int32_t device;
     uint32_t cu_count;      /* compute units */
