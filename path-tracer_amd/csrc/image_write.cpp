@@ -1,5 +1,8 @@
 // image_write.cpp - encoders for the output image of `pt-render render`:
-// PNG, baseline JPEG and lossless WebP, from RGB8 rows.
+// PNG, baseline JPEG and lossless WebP, from RGB8 rows; and, for scene
+// export, RGBA PNG textures, uncompressed FLOAT OpenEXR skies and exact
+// Radiance RGBE skies (run-length coded scanlines, refused unless every value
+// decodes back unchanged).
 //
 // The reference saves its tone-mapped RGB8 image with image::save_buffer
 // (headless.rs:163-170) and picks the format from the file extension: .png,
