@@ -309,16 +309,19 @@ pt_status pt_scene_create(pt_context* c, const pt_scene_desc* d, pt_scene** out)
         opaque[i] = d->materials[d->attributes[i].material].alpha_mode == PT_ALPHA_MODE_OPAQUE ? 1 : 0;
         all_opaque = all_opaque && opaque[i];
     }
-    // traversal node copy with each node's child code precomputed (see DevScene::tnode)
-    std::vector<float4> tnodes(2 * (size_t)d->bvh_node_count);
+    // traversal node copy with each node's child code precomputed (see DevScene::tnode).
+    // Child pairs start at odd node indices (the root is alone at 0), so a
+    // 32-byte lead puts every 64-byte pair record in one 64-byte segment of
+    // the 256-byte-aligned allocation: one L1 data-return beat per record.
+    std::vector<float4> tnodes(ptd::TNODE_LEAD + 2 * (size_t)d->bvh_node_count, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     for (uint32_t i = 0; i < d->bvh_node_count; ++i) {
         const PtBvhNode& n = d->bvh_nodes[i];
         uint32_t code = ptd::child_code(i, n.left_first, n.tri_count);
         float cf, tf;
         std::memcpy(&cf, &code, 4);
         std::memcpy(&tf, &n.tri_count, 4);
-        tnodes[2 * (size_t)i] = make_float4(n.aabb_min[0], n.aabb_min[1], n.aabb_min[2], cf);
-        tnodes[2 * (size_t)i + 1] = make_float4(n.aabb_max[0], n.aabb_max[1], n.aabb_max[2], tf);
+        tnodes[ptd::TNODE_LEAD + 2 * (size_t)i] = make_float4(n.aabb_min[0], n.aabb_min[1], n.aabb_min[2], cf);
+        tnodes[ptd::TNODE_LEAD + 2 * (size_t)i + 1] = make_float4(n.aabb_max[0], n.aabb_max[1], n.aabb_max[2], tf);
     }
     std::vector<float4> sky((size_t)d->skybox.width * d->skybox.height);
     std::memcpy(sky.data(), d->skybox_rgba, sky.size() * sizeof(float4));
@@ -336,7 +339,8 @@ pt_status pt_scene_create(pt_context* c, const pt_scene_desc* d, pt_scene** out)
         v.attr = attr;
         v.mat = mat;
         v.bvh = bvh;
-        v.tnode = tn;
+        s->tnode_alloc = tn;
+        v.tnode = tn ? tn + ptd::TNODE_LEAD : nullptr;
         v.srgb = srgb;
         v.linear = lin;
         v.tri_opaque = opq;
@@ -391,7 +395,7 @@ pt_status pt_scene_destroy(pt_scene* s) {
     (void)hipFree((void*)v.attr);
     (void)hipFree((void*)v.mat);
     (void)hipFree((void*)v.bvh);
-    (void)hipFree((void*)v.tnode);
+    (void)hipFree((void*)s->tnode_alloc);
     (void)hipFree((void*)v.srgb);
     (void)hipFree((void*)v.linear);
     (void)hipFree((void*)v.tri_opaque);

@@ -28,7 +28,9 @@ struct DevScene {
     const PtBvhNode* bvh;
     // traversal copy of the nodes: node i is tnode[2i] = (aabb_min, code),
     // tnode[2i + 1] = (aabb_max, tri_count), with the code of child_code()
-    // precomputed, so a child pair read needs no decoding
+    // precomputed, so a child pair read needs no decoding. The array starts
+    // TNODE_LEAD entries into its allocation, which puts every child pair
+    // (odd first index) on a 64-byte boundary.
     const float4* tnode;
     const uint8_t* srgb;       // sRGB atlas layers, RGBA8
     const uint8_t* linear;     // linear atlas layers, RGBA8
@@ -45,6 +47,8 @@ struct DevScene {
     uint32_t all_opaque;
     uint32_t pad;
 };
+
+constexpr size_t TNODE_LEAD = 2;  // float4 entries before node 0 (32 bytes)
 
 // ------------------------------------------------------------------ RNG
 struct Rng {
@@ -357,6 +361,98 @@ struct TraversalCounters {
     uint32_t pairs, tris;
 };
 
+// Leaf step of the closest-hit walk: the triangles of leaf `cur`, strictly
+// nearer hits accepted, alpha-tested candidates drawing as passes_through does.
+// PAIR: a leaf of one or two triangles requests both before the first test (a
+// one-triangle leaf loads its triangle twice), then tests them in order.
+template <bool COUNT, bool PAIR>
+__device__ __forceinline__ void leaf_closest(const DevScene& s, uint32_t cur, f3 o, const Watertight& wt, Rng& rng,
+                                             float& hit_t, int32_t& hit, float& hit_v, float& hit_w,
+                                             TraversalCounters& ctr) {
+    if (PAIR && (cur & CODE_SMALL_LEAF)) {
+        const uint32_t t0 = (cur >> 1) & 0x3fffffffu, t1 = t0 + (cur & 1u);
+        const PtTriangleGeometry g0 = s.geo[t0], g1 = s.geo[t1];
+        float tt, bv, bw;
+        if (COUNT) ++ctr.tris;
+        if (tri_test(g0.v, o, wt, hit_t, tt, bv, bw) && !passes_through(s, t0, bv, bw, rng)) {
+            hit_t = tt;
+            hit = (int32_t)t0;
+            hit_v = bv;
+            hit_w = bw;
+        }
+        if (cur & 1u) {
+            if (COUNT) ++ctr.tris;
+            if (tri_test(g1.v, o, wt, hit_t, tt, bv, bw) && !passes_through(s, t1, bv, bw, rng)) {
+                hit_t = tt;
+                hit = (int32_t)t1;
+                hit_v = bv;
+                hit_w = bw;
+            }
+        }
+        return;
+    }
+    uint32_t first, count;
+    if (cur & CODE_SMALL_LEAF) {
+        first = (cur >> 1) & 0x3fffffffu;
+        count = (cur & 1u) + 1u;
+    } else {
+        PtBvhNode n = s.bvh[cur & 0x3fffffffu];
+        first = n.left_first;
+        count = n.tri_count;
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t ti = first + i;
+        if (COUNT) ++ctr.tris;
+        float tt, bv, bw;
+        if (tri_test(s.geo[ti].v, o, wt, hit_t, tt, bv, bw)) {
+            if (passes_through(s, ti, bv, bw, rng)) continue;
+            hit_t = tt;
+            hit = (int32_t)ti;
+            hit_v = bv;
+            hit_w = bw;
+        }
+    }
+}
+
+// Leaf step of the any-hit walk: true when a triangle of leaf `cur` other
+// than light_tri blocks the ray (an alpha-tested one unless it passes through).
+template <bool COUNT, bool PAIR>
+__device__ __forceinline__ bool leaf_any(const DevScene& s, uint32_t cur, f3 o, const Watertight& wt, float hit_t,
+                                         int32_t light_tri, Rng& rng, TraversalCounters& ctr) {
+    if (PAIR && (cur & CODE_SMALL_LEAF)) {
+        // as in the closest-hit leaf: both triangles requested, then tested in order
+        const uint32_t t0 = (cur >> 1) & 0x3fffffffu, t1 = t0 + (cur & 1u);
+        const PtTriangleGeometry g0 = s.geo[t0], g1 = s.geo[t1];
+        float tt, bv, bw;
+        if ((int32_t)t0 != light_tri) {
+            if (COUNT) ++ctr.tris;
+            if (tri_test(g0.v, o, wt, hit_t, tt, bv, bw) && !passes_through(s, t0, bv, bw, rng)) return true;
+        }
+        if ((cur & 1u) && (int32_t)t1 != light_tri) {
+            if (COUNT) ++ctr.tris;
+            if (tri_test(g1.v, o, wt, hit_t, tt, bv, bw) && !passes_through(s, t1, bv, bw, rng)) return true;
+        }
+        return false;
+    }
+    uint32_t first, count;
+    if (cur & CODE_SMALL_LEAF) {
+        first = (cur >> 1) & 0x3fffffffu;
+        count = (cur & 1u) + 1u;
+    } else {
+        PtBvhNode n = s.bvh[cur & 0x3fffffffu];
+        first = n.left_first;
+        count = n.tri_count;
+    }
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t ti = first + i;
+        if ((int32_t)ti == light_tri) continue;
+        if (COUNT) ++ctr.tris;
+        float tt, bv, bw;
+        if (tri_test(s.geo[ti].v, o, wt, hit_t, tt, bv, bw) && !passes_through(s, ti, bv, bw, rng)) return true;
+    }
+    return false;
+}
+
 // Closest hit (trace.slang:785-884). Visits children nearer-first with ties
 // to the right child, accepts strictly nearer hits, so the hit index equals
 // the reference's on the same BVH. rng is advanced by alpha-tested
@@ -399,50 +495,8 @@ __device__ __forceinline__ int32_t closest_hit(const DevScene& s, f3 o, f3 d, f3
                 cur = cr;
                 continue;
             }
-        } else if (PAIR && (cur & CODE_SMALL_LEAF)) {
-            // one or two triangles: both are requested before the first test
-            // (a one-triangle leaf loads its triangle twice), then tested in order
-            const uint32_t t0 = (cur >> 1) & 0x3fffffffu, t1 = t0 + (cur & 1u);
-            const PtTriangleGeometry g0 = s.geo[t0], g1 = s.geo[t1];
-            float tt, bv, bw;
-            if (COUNT) ++ctr.tris;
-            if (tri_test(g0.v, o, wt, hit_t, tt, bv, bw) && !passes_through(s, t0, bv, bw, rng)) {
-                hit_t = tt;
-                hit = (int32_t)t0;
-                hit_v = bv;
-                hit_w = bw;
-            }
-            if (cur & 1u) {
-                if (COUNT) ++ctr.tris;
-                if (tri_test(g1.v, o, wt, hit_t, tt, bv, bw) && !passes_through(s, t1, bv, bw, rng)) {
-                    hit_t = tt;
-                    hit = (int32_t)t1;
-                    hit_v = bv;
-                    hit_w = bw;
-                }
-            }
         } else {
-            uint32_t first, count;
-            if (cur & CODE_SMALL_LEAF) {
-                first = (cur >> 1) & 0x3fffffffu;
-                count = (cur & 1u) + 1u;
-            } else {
-                PtBvhNode n = s.bvh[cur & 0x3fffffffu];
-                first = n.left_first;
-                count = n.tri_count;
-            }
-            for (uint32_t i = 0; i < count; ++i) {
-                uint32_t ti = first + i;
-                if (COUNT) ++ctr.tris;
-                float tt, bv, bw;
-                if (tri_test(s.geo[ti].v, o, wt, hit_t, tt, bv, bw)) {
-                    if (passes_through(s, ti, bv, bw, rng)) continue;
-                    hit_t = tt;
-                    hit = (int32_t)ti;
-                    hit_v = bv;
-                    hit_w = bw;
-                }
-            }
+            leaf_closest<COUNT, PAIR>(s, cur, o, wt, rng, hit_t, hit, hit_v, hit_w, ctr);
         }
         if (st.sp == 0u) break;
         cur = st.pop();
@@ -481,45 +535,9 @@ __device__ __forceinline__ bool occluded(const DevScene& s, f3 o, f3 d, float hi
                 cur = cr;
                 continue;
             }
-        } else if (PAIR && (cur & CODE_SMALL_LEAF)) {
-            // as in closest_hit: both triangles requested, then tested in order
-            const uint32_t t0 = (cur >> 1) & 0x3fffffffu, t1 = t0 + (cur & 1u);
-            const PtTriangleGeometry g0 = s.geo[t0], g1 = s.geo[t1];
-            float tt, bv, bw;
-            if ((int32_t)t0 != light_tri) {
-                if (COUNT) ++ctr.tris;
-                if (tri_test(g0.v, o, wt, hit_t, tt, bv, bw) && !passes_through(s, t0, bv, bw, rng)) {
-                    st.sp = 0;
-                    return true;
-                }
-            }
-            if ((cur & 1u) && (int32_t)t1 != light_tri) {
-                if (COUNT) ++ctr.tris;
-                if (tri_test(g1.v, o, wt, hit_t, tt, bv, bw) && !passes_through(s, t1, bv, bw, rng)) {
-                    st.sp = 0;
-                    return true;
-                }
-            }
-        } else {
-            uint32_t first, count;
-            if (cur & CODE_SMALL_LEAF) {
-                first = (cur >> 1) & 0x3fffffffu;
-                count = (cur & 1u) + 1u;
-            } else {
-                PtBvhNode n = s.bvh[cur & 0x3fffffffu];
-                first = n.left_first;
-                count = n.tri_count;
-            }
-            for (uint32_t i = 0; i < count; ++i) {
-                uint32_t ti = first + i;
-                if ((int32_t)ti == light_tri) continue;
-                if (COUNT) ++ctr.tris;
-                float tt, bv, bw;
-                if (tri_test(s.geo[ti].v, o, wt, hit_t, tt, bv, bw) && !passes_through(s, ti, bv, bw, rng)) {
-                    st.sp = 0;
-                    return true;
-                }
-            }
+        } else if (leaf_any<COUNT, PAIR>(s, cur, o, wt, hit_t, light_tri, rng, ctr)) {
+            st.sp = 0;
+            return true;
         }
         if (st.sp == 0u) break;
         cur = st.pop();
@@ -749,12 +767,40 @@ struct Bsdf {
     float ltc_a, ltc_b, ltc_c, ltc_d;
     float bxx, bxy;  // LTC basis row X = (bxx, bxy, 0); Y = (-bxy, bxx, 0); Z = (0, 0, 1)
     float P_u, P_c, p_ggx, p_diffuse;
+    // evaluate_lobes' factors that depend on wo and the material only
+    // (lobe_wo_terms), computed once per hit for both of its calls
+    float G1;        // 1 / (1 + Lambda(wo))
+    f3 ss_scale;     // rho / pi * A_F of the single-scattering EON lobe
+    f3 ms_wo;        // rho_ms / pi * max(1e-7, 1 - E_F(mu_o)) of the multiple-scattering lobe
+    float ms_den;    // max(1e-7, 1 - E_F average)
+    float detM_sq;   // det(M)^2 of the LTC
+    float s_pdf;     // the LTC clipping split, also the sampling lerp weight
 };
+
+// The factors of evaluate_lobes (trace.slang:229-315) that depend only on wo
+// and the material, in the shader's expressions: the shader evaluates them in
+// each of its two lobe evaluations per hit, here they are kept in the Bsdf.
+__device__ __forceinline__ void lobe_wo_terms(Bsdf& b, f3 wo) {
+    const f3 rho = b.base_color;
+    const float r = b.base_diffuse_roughness;
+    b.G1 = 1.0f / b.one_plus_lambda_wo;
+    float AF = 1.0f / ptm::fma_(0.28779340921f, r, 1.0f);
+    b.ss_scale = rho * ptm::INV_PI * AF;
+    float EFo = e_fon_exact(wo.z, r);
+    float avgEF = AF * ptm::fma_(0.07248821245f, r, 1.0f);
+    f3 rho_ms = rho * rho * avgEF / (1.0f - rho * (1.0f - avgEF));
+    b.ms_wo = (rho_ms * ptm::INV_PI) * fmaxf(1.0e-7f, 1.0f - EFo);
+    b.ms_den = fmaxf(1.0e-7f, 1.0f - avgEF);
+    const float a = b.ltc_a, bb = b.ltc_b, c = b.ltc_c, d = b.ltc_d;
+    float detM = c * ptm::fma_(-bb, d, a);
+    b.detM_sq = detM * detM;
+    b.s_pdf = ptm::fma_(0.5f, ptm::rsqrt_(ptm::fma_(d, d, 1.0f)), 0.5f);
+}
 
 __device__ __forceinline__ void evaluate_lobes(const Bsdf& b, f3 wo, f3 wi, f3& f_total, float& pdf_total) {
     f3 H = ptm::normalize(wo + wi);
     float HdotW = ptm::dot(H, wo);
-    float G1 = 1.0f / b.one_plus_lambda_wo;
+    const float G1 = b.G1;
     float G2 = 1.0f / (b.one_plus_lambda_wo + ggx_lambda(wi, b.ax, b.ay));
     float D_helper = (H.x * H.x) / (b.ax * b.ax) + (H.y * H.y) / (b.ay * b.ay) + H.z * H.z;
     float D = ptm::INV_PI / (b.ax * b.ay * D_helper * D_helper);
@@ -767,17 +813,12 @@ __device__ __forceinline__ void evaluate_lobes(const Bsdf& b, f3 wo, f3 wi, f3& 
     f3 F82 = b.F0_conductor - fifth_power(omh) * ptm::fma3(fa, fb, fc);
     f3 f_conductor = ptm::saturate3(b.specular_weight * F82) * ts;
     f3 f_dielectric = b.specular_color * fresnel_exact(b.sim, HdotW) * ts;
-    const f3 rho = b.base_color;
     const float r = b.base_diffuse_roughness;
     float s_f = wi.x * wo.x + wi.y * wo.y;
     float sovertF = s_f > 0.0f ? s_f / fmaxf(wo.z, wi.z) : s_f;
-    float AF = 1.0f / ptm::fma_(0.28779340921f, r, 1.0f);
-    f3 f_ss = rho * ptm::INV_PI * AF * ptm::fma_(r, sovertF, 1.0f);
-    float EFo = e_fon_exact(wo.z, r);
+    f3 f_ss = b.ss_scale * ptm::fma_(r, sovertF, 1.0f);
     float EFi = e_fon_exact(wi.z, r);
-    float avgEF = AF * ptm::fma_(0.07248821245f, r, 1.0f);
-    f3 rho_ms = rho * rho * avgEF / (1.0f - rho * (1.0f - avgEF));
-    f3 f_ms = (rho_ms * ptm::INV_PI) * fmaxf(1.0e-7f, 1.0f - EFo) * fmaxf(1.0e-7f, 1.0f - EFi) / fmaxf(1.0e-7f, 1.0f - avgEF);
+    f3 f_ms = b.ms_wo * fmaxf(1.0e-7f, 1.0f - EFi) / b.ms_den;
     f3 f_diffuse = b.base_weight * (f_ss + f_ms);
     f3 f_glossy_diffuse = f_dielectric + (1.0f - b.E_dielectric) * f_diffuse;
     f_total = ptm::lerp3(f_glossy_diffuse, f_conductor, b.base_metalness);
@@ -785,11 +826,9 @@ __device__ __forceinline__ void evaluate_lobes(const Bsdf& b, f3 wo, f3 wi, f3& 
     f3 bx = {b.bxx, b.bxy, 0.0f}, by = {-b.bxy, b.bxx, 0.0f}, bz = {0.0f, 0.0f, 1.0f};
     f3 wl = {ptm::dot(bx, wi), ptm::dot(by, wi), ptm::dot(bz, wi)};
     const float a = b.ltc_a, bb = b.ltc_b, c = b.ltc_c, d = b.ltc_d;
-    float detM = c * ptm::fma_(-bb, d, a);
     f3 wh = {c * (wl.x - bb * wl.z), (a - bb * d) * wl.y, -c * (d * wl.x - a * wl.z)};
     float lenSqr = ptm::dot(wh, wh);
-    float s_pdf = ptm::fma_(0.5f, ptm::rsqrt_(ptm::fma_(d, d, 1.0f)), 0.5f);
-    float pdf_c = (detM * detM) / (lenSqr * lenSqr) * fmaxf(wh.z, 0.0f) / (ptm::PI * s_pdf);
+    float pdf_c = b.detM_sq / (lenSqr * lenSqr) * fmaxf(wh.z, 0.0f) / (ptm::PI * b.s_pdf);
     float pdf_eon = b.P_u * 0.15915494309f + b.P_c * pdf_c;
     pdf_total = b.p_ggx * pdf_ggx + b.p_diffuse * pdf_eon;
 }
@@ -932,6 +971,7 @@ __device__ __forceinline__ void setup_surface(const DevScene& s, uint32_t hit, f
     b.p_diffuse = 1.0f - b.p_ggx;
     b.P_u = ptm::pow_(r, 0.1f) * (0.162925f + (-0.372058f + (0.538233f - 0.290822f * z) * z) * z);
     b.P_c = 1.0f - b.P_u;
+    lobe_wo_terms(b, wo);
 }
 
 // BSDF direction sample (trace.slang:1212-1260); consumes 3 draws.
@@ -949,7 +989,7 @@ __device__ __forceinline__ f3 sample_bsdf(const Bsdf& b, f3 wo, Rng& rng) {
         float u1 = (rand1 - b.P_u) / b.P_c;
         float R = sqrtf(u1);
         float y = R * ptm::sin_(two_pi_rand);
-        float sp = ptm::fma_(0.5f, ptm::rsqrt_(ptm::fma_(b.ltc_d, b.ltc_d, 1.0f)), 0.5f);
+        const float sp = b.s_pdf;
         float x = -ptm::lerp_(sqrtf(ptm::fma_(-y, y, 1.0f)), R * ptm::cos_(two_pi_rand), sp);
         f3 wh = {x, y, sqrtf(fmaxf(1.0f - (x * x + y * y), 0.0f))};
         f3 wu = {b.ltc_a * wh.x + b.ltc_b * wh.z, b.ltc_c * wh.y, b.ltc_d * wh.x + wh.z};

@@ -41,6 +41,7 @@ struct Context {
 struct Scene {
     Context* ctx = nullptr;
     ptd::DevScene view{};
+    const float4* tnode_alloc = nullptr;  // allocation behind view.tnode (which starts TNODE_LEAD entries in)
     uint32_t bvh_depth = 0;
     uint32_t triangle_count = 0, material_count = 0, bvh_node_count = 0;
     uint32_t scene_alias_capacity = 0, skybox_alias_count = 0;
