@@ -65,7 +65,7 @@ _SIG = [
     ("oracle_math", C.c_int, [C.c_int, _p, _p, _p, C.c_uint32]),
 ]
 
-MATH_FUNCS = {"sin": 0, "cos": 1, "acos": 2, "atan2": 3, "pow": 4, "log2": 5, "exp2": 6}
+MATH_FUNCS = {"sin": 0, "cos": 1, "acos": 2, "atan2": 3, "pow": 4, "log2": 5, "exp2": 6, "sincos_sin": 7, "sincos_cos": 8}
 
 
 def detmath(name, x, y=None):

@@ -1179,9 +1179,11 @@ float oracle_rand(uint32_t* state) {
 uint32_t oracle_seed(uint32_t px, uint32_t py, uint32_t sample_count) { return seed_of(px, py, sample_count); }
 
 // Evaluate one deterministic-libm function over arrays (fn: 0 sin, 1 cos,
-// 2 acos, 3 atan2(x, y), 4 pow(x, y), 5 log2, 6 exp2).
+// 2 acos, 3 atan2(x, y), 4 pow(x, y), 5 log2, 6 exp2, 7 and 8: the sine and
+// the cosine from the kernels' one-reduction sincos_).
 int oracle_math(int fn, const float* x, const float* y, float* out, uint32_t n) {
     for (uint32_t i = 0; i < n; ++i) {
+        float sn, cs;
         switch (fn) {
             case 0: out[i] = ptm::sin_(x[i]); break;
             case 1: out[i] = ptm::cos_(x[i]); break;
@@ -1190,6 +1192,8 @@ int oracle_math(int fn, const float* x, const float* y, float* out, uint32_t n) 
             case 4: out[i] = ptm::pow_(x[i], y[i]); break;
             case 5: out[i] = ptm::log2_(x[i]); break;
             case 6: out[i] = ptm::exp2_(x[i]); break;
+            case 7: ptm::sincos_(x[i], sn, cs); out[i] = sn; break;
+            case 8: ptm::sincos_(x[i], sn, cs); out[i] = cs; break;
             default: return 1;
         }
     }

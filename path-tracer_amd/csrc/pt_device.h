@@ -243,7 +243,9 @@ __device__ __forceinline__ LightSample sample_light(const DevScene& s, f3 origin
         float shs = ptm::lerp_(a * a, b * b, rng.next());
         float cos_theta = 1.0f - 2.0f * shs;
         float sin_theta = 2.0f * sqrtf(fmaxf(0.0f, shs * (1.0f - shs)));
-        r.direction = f3{sin_theta * ptm::cos_(phi), north ? cos_theta : -cos_theta, sin_theta * ptm::sin_(phi)};
+        float sn, cs;
+        ptm::sincos_(phi, sn, cs);
+        r.direction = f3{sin_theta * cs, north ? cos_theta : -cos_theta, sin_theta * sn};
         r.distance = ptm::FLT_MAX_;
         r.pdf = skybox_light_pdf(s, r.direction, py);
         r.emission = evaluate_skybox(s, r.direction);
@@ -738,8 +740,10 @@ __device__ __forceinline__ f3 sample_vndf(f3 wo, float ax, float ay, float r, fl
     f3 wo_warped = ptm::normalize(f3{wo.x * ax, wo.y * ay, wo.z});
     float z = ptm::fma_(1.0f - r, 1.0f + wo_warped.z, -wo_warped.z);
     float sin_theta = sqrtf(ptm::saturate_(ptm::fma_(-z, z, 1.0f)));
-    float x = sin_theta * ptm::cos_(two_pi_r);
-    float y = sin_theta * ptm::sin_(two_pi_r);
+    float sn, cs;
+    ptm::sincos_(two_pi_r, sn, cs);
+    float x = sin_theta * cs;
+    float y = sin_theta * sn;
     f3 h = f3{x, y, z} + wo_warped;
     return ptm::normalize(f3{h.x * ax, h.y * ay, h.z});
 }
@@ -984,13 +988,17 @@ __device__ __forceinline__ f3 sample_bsdf(const Bsdf& b, f3 wo, Rng& rng) {
     } else if (rand1 < b.P_u) {
         float u1 = rand1 / b.P_u;
         float sinTheta = sqrtf(1.0f - u1 * u1);
-        wi = f3{sinTheta * ptm::cos_(two_pi_rand), sinTheta * ptm::sin_(two_pi_rand), u1};
+        float sn, cs;
+        ptm::sincos_(two_pi_rand, sn, cs);
+        wi = f3{sinTheta * cs, sinTheta * sn, u1};
     } else {
         float u1 = (rand1 - b.P_u) / b.P_c;
         float R = sqrtf(u1);
-        float y = R * ptm::sin_(two_pi_rand);
+        float sn, cs;
+        ptm::sincos_(two_pi_rand, sn, cs);
+        float y = R * sn;
         const float sp = b.s_pdf;
-        float x = -ptm::lerp_(sqrtf(ptm::fma_(-y, y, 1.0f)), R * ptm::cos_(two_pi_rand), sp);
+        float x = -ptm::lerp_(sqrtf(ptm::fma_(-y, y, 1.0f)), R * cs, sp);
         f3 wh = {x, y, sqrtf(fmaxf(1.0f - (x * x + y * y), 0.0f))};
         f3 wu = {b.ltc_a * wh.x + b.ltc_b * wh.z, b.ltc_c * wh.y, b.ltc_d * wh.x + wh.z};
         f3 bx = {b.bxx, b.bxy, 0.0f}, by = {-b.bxy, b.bxx, 0.0f}, bz = {0.0f, 0.0f, 1.0f};

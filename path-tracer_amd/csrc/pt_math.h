@@ -111,6 +111,18 @@ PT_HD float cos_(float x) {
     float c = (q & 1) ? sin_poly(r) : cos_poly(r);
     return ((q + 1) & 2) ? -c : c;
 }
+// sin_(x) and cos_(x) bit for bit from one reduction: both polynomials are
+// evaluated and the quadrant only selects between them, so the GPU build has
+// no lane-divergent branch here
+PT_HD void sincos_(float x, float& s, float& c) {
+    int q;
+    const float r = reduce_pio2(x, q);
+    const float sp = sin_poly(r), cp = cos_poly(r);
+    const float s0 = (q & 1) ? cp : sp;
+    const float c0 = (q & 1) ? sp : cp;
+    s = (q & 2) ? -s0 : s0;
+    c = ((q + 1) & 2) ? -c0 : c0;
+}
 
 // ---------------------------------------------------------------- atan / atan2
 PT_HD float atan_poly(float u) {  // |u| <= tan(pi/8)

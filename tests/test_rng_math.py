@@ -105,3 +105,19 @@ def test_detmath_exact_points():
     assert oracle.detmath("acos", np.array([1.0], np.float32))[0] == 0.0
     assert abs(oracle.detmath("acos", np.array([-1.0], np.float32))[0] - np.float32(np.pi)) <= np.spacing(np.float32(np.pi))
     assert oracle.detmath("exp2", np.array([10.0], np.float32))[0] == 1024.0
+
+
+def test_sincos_equals_sin_and_cos():
+    """The kernels' one-reduction sincos_ gives sin_ and cos_ bit for bit: every
+    float angle of [0, 2pi) sampled densely (the sampling angles 2 pi rand), and
+    angles across all quadrants up to the reduction's range."""
+    rng = np.random.default_rng(7)
+    x = np.concatenate([
+        np.float32(2.0 * np.pi) * (np.arange(1 << 20, dtype=np.float32) / np.float32(1 << 20)),
+        rng.uniform(-1e4, 1e4, 1 << 18).astype(np.float32),
+        np.float32(np.pi / 4) * np.arange(-64, 65, dtype=np.float32),
+    ])
+    for part, ref in (("sincos_sin", "sin"), ("sincos_cos", "cos")):
+        got = oracle.detmath(part, x).view(np.uint32)
+        want = oracle.detmath(ref, x).view(np.uint32)
+        assert np.array_equal(got, want), part
