@@ -365,15 +365,17 @@ struct TraversalCounters {
 
 // Leaf step of the closest-hit walk: the triangles of leaf `cur`, strictly
 // nearer hits accepted, alpha-tested candidates drawing as passes_through does.
-// PAIR: a leaf of one or two triangles requests both before the first test (a
-// one-triangle leaf loads its triangle twice), then tests them in order.
+// PAIR: a leaf of two triangles requests both before the first test, then
+// tests them in order.
 template <bool COUNT, bool PAIR>
 __device__ __forceinline__ void leaf_closest(const DevScene& s, uint32_t cur, f3 o, const Watertight& wt, Rng& rng,
                                              float& hit_t, int32_t& hit, float& hit_v, float& hit_w,
                                              TraversalCounters& ctr) {
     if (PAIR && (cur & CODE_SMALL_LEAF)) {
-        const uint32_t t0 = (cur >> 1) & 0x3fffffffu, t1 = t0 + (cur & 1u);
-        const PtTriangleGeometry g0 = s.geo[t0], g1 = s.geo[t1];
+        const uint32_t t0 = (cur >> 1) & 0x3fffffffu, t1 = t0 + 1u;
+        const PtTriangleGeometry g0 = s.geo[t0];
+        PtTriangleGeometry g1;
+        if (cur & 1u) g1 = s.geo[t1];
         float tt, bv, bw;
         if (COUNT) ++ctr.tris;
         if (tri_test(g0.v, o, wt, hit_t, tt, bv, bw) && !passes_through(s, t0, bv, bw, rng)) {
@@ -423,8 +425,10 @@ __device__ __forceinline__ bool leaf_any(const DevScene& s, uint32_t cur, f3 o, 
                                          int32_t light_tri, Rng& rng, TraversalCounters& ctr) {
     if (PAIR && (cur & CODE_SMALL_LEAF)) {
         // as in the closest-hit leaf: both triangles requested, then tested in order
-        const uint32_t t0 = (cur >> 1) & 0x3fffffffu, t1 = t0 + (cur & 1u);
-        const PtTriangleGeometry g0 = s.geo[t0], g1 = s.geo[t1];
+        const uint32_t t0 = (cur >> 1) & 0x3fffffffu, t1 = t0 + 1u;
+        const PtTriangleGeometry g0 = s.geo[t0];
+        PtTriangleGeometry g1;
+        if (cur & 1u) g1 = s.geo[t1];
         float tt, bv, bw;
         if ((int32_t)t0 != light_tri) {
             if (COUNT) ++ctr.tris;
