@@ -340,8 +340,8 @@ typedef struct pt_wave_config {
                                   1: lane refill (a lane that finishes starts its next ray at once);
                                   2: threshold refill (idle lanes start their next rays together
                                      once fewer than half of the wave is still traversing) */
-    uint32_t streams;          /* 1, or 2: chunks alternate between two HIP streams with their own
-                                  path state, so one chunk's shading overlaps the other's traversal;
+    uint32_t streams;          /* 1, or 2: chunks alternate between the context's two HIP streams,
+                                  each with its own path state, so one chunk's shading overlaps the other's traversal;
                                   0: automatic (2 when a render spans three chunks or more, or is at
                                   least lane_split_paths and fits two chunks) */
     uint32_t trace_waves_per_cu;  /* traversal grid: one-wave blocks per CU, 16..2048 (default 512) */
@@ -388,8 +388,8 @@ pt_status pt_autotune_batch(pt_tracer* const* tracers, uint32_t count, const uin
  * admits queued jobs while slots and the HBM budget allow, and traces them
  * with continuous batching. Each round is one wavefront chunk made of pixel
  * blocks from every active job; jobs submitted meanwhile join the next
- * round. Rounds are pipelined over two lanes (HIP streams with their own
- * path state): while one round runs, the next is built and launched on the
+ * round. Rounds are pipelined over two lanes (the context's two HIP streams,
+ * each lane with its own path state): while one round runs, the next is built and launched on the
  * other lane, so a round's last, small bounces overlap the next round's
  * first, large ones. A job's pixels always take all of their samples within
  * one round, in sample order, so its image is bit-identical however it is

@@ -231,7 +231,9 @@ pt_status pt_context_create(int device, pt_context** out) {
     c->hbm_bytes = (uint64_t)prop.totalGlobalMem;
     std::snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->lane_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return hip_error(e, "hipStreamCreate");
     }
@@ -254,12 +256,15 @@ pt_status pt_context_create(int device, pt_context** out) {
 pt_status pt_context_destroy(pt_context* c) {
     if (!c) return PT_OK;
     (void)hipSetDevice(c->device);
-    const hipError_t e = c->stream ? hipStreamSynchronize(c->stream) : hipSuccess;
+    hipError_t e = c->stream ? hipStreamSynchronize(c->stream) : hipSuccess;
+    const hipError_t el = c->lane_stream ? hipStreamSynchronize(c->lane_stream) : hipSuccess;
+    if (e == hipSuccess) e = el;
     (void)hipFree(c->ovf);
     (void)hipFree(c->srgb_lut);
     (void)hipFree(c->counters);
     delete c->batch_wave;
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->lane_stream) (void)hipStreamDestroy(c->lane_stream);
     delete c;
     return e == hipSuccess ? PT_OK : hip_error(e, "work pending at context destruction");
 }
@@ -279,6 +284,8 @@ pt_status pt_context_get_info(const pt_context* c, pt_context_info* info) {
 pt_status pt_context_synchronize(pt_context* c) {
     if (!c) return set_error(PT_ERR_INVALID, "null context");
     hipError_t e = hipStreamSynchronize(c->stream);
+    const hipError_t el = hipStreamSynchronize(c->lane_stream);
+    if (e == hipSuccess) e = el;
     return e == hipSuccess ? PT_OK : hip_error(e, "hipStreamSynchronize");
 }
 

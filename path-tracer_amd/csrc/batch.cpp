@@ -289,8 +289,9 @@ constexpr int SERVER_LANES = 2;
 struct pt_server {
     Context* ctx = nullptr;
     pt_server_config cfg{};
-    // two lanes with their own stream and path state: rounds alternate between
-    // them, and the next round is launched while the current one runs
+    // two lanes with their own path state on the context's two streams: rounds
+    // alternate between them, and the next round is launched while the current
+    // one runs
     hipStream_t lane_stream[SERVER_LANES] = {nullptr, nullptr};
     hipEvent_t lane_done[SERVER_LANES] = {nullptr, nullptr};
     bool lane_busy[SERVER_LANES] = {false, false};
@@ -558,9 +559,10 @@ pt_status pt_server_create(pt_context* c, const pt_server_config* cfg, pt_server
     arena = std::max<uint64_t>(arena & ~(uint64_t)255, 256);
     s->cfg.job_arena_bytes = arena;
     hipError_t e = hipMalloc(&s->arena_mem, arena);
+    s->lane_stream[0] = c->stream;
+    s->lane_stream[1] = c->lane_stream;
     for (int l = 0; l < SERVER_LANES && e == hipSuccess; ++l) {
-        e = hipStreamCreateWithFlags(&s->lane_stream[l], hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&s->lane_done[l], hipEventDisableTiming);
+        e = hipEventCreateWithFlags(&s->lane_done[l], hipEventDisableTiming);
         s->wave[l].ctx = c;
         s->wave[l].stream_override = s->lane_stream[l];
         s->wave[l].one_lane = true;
@@ -568,7 +570,6 @@ pt_status pt_server_create(pt_context* c, const pt_server_config* cfg, pt_server
     if (e != hipSuccess) {
         for (int l = 0; l < SERVER_LANES; ++l) {
             if (s->lane_done[l]) (void)hipEventDestroy(s->lane_done[l]);
-            if (s->lane_stream[l]) (void)hipStreamDestroy(s->lane_stream[l]);
         }
         if (s->arena_mem) (void)hipFree(s->arena_mem);
         delete s;
@@ -607,9 +608,7 @@ pt_status pt_server_destroy(pt_server* s) {
         (void)hipEventDestroy(s->lane_done[l]);
     }
     (void)hipFree(s->arena_mem);
-    hipStream_t streams[SERVER_LANES] = {s->lane_stream[0], s->lane_stream[1]};
     delete s;  // the lanes' path state is released here, with their work finished
-    for (hipStream_t st : streams) (void)hipStreamDestroy(st);
     return pending == hipSuccess ? PT_OK : hip_error(pending, "work pending at server destruction");
 }
 

@@ -910,7 +910,7 @@ EventLog::~EventLog() {
 // Teardown has no caller to report to: failures of the releases are ignored.
 Wavefront::~Wavefront() {
     if (partner) {
-        (void)hipStreamSynchronize(partner->own_stream);
+        (void)hipStreamSynchronize(partner->stream_override);
         delete partner;
     }
     if (arena) (void)hipFree(arena);
@@ -919,7 +919,6 @@ Wavefront::~Wavefront() {
     if (staging) (void)hipHostFree(staging);
     if (ev_start) (void)hipEventDestroy(ev_start);
     if (ev_done) (void)hipEventDestroy(ev_done);
-    if (own_stream) (void)hipStreamDestroy(own_stream);
 }
 
 pt_status Wavefront::ensure_overflow(size_t need, hipStream_t stream) {
@@ -1033,14 +1032,13 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
             partner = new (std::nothrow) Wavefront();
             if (!partner) return set_error(PT_ERR_MEMORY, "out of host memory");
             partner->ctx = ctx;
-            hipError_t e = hipStreamCreateWithFlags(&partner->own_stream, hipStreamNonBlocking);
-            if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_start, hipEventDisableTiming);
+            partner->stream_override = ctx->lane_stream;
+            hipError_t e = hipEventCreateWithFlags(&ev_start, hipEventDisableTiming);
             if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_done, hipEventDisableTiming);
             if (e != hipSuccess) return hip_error(e, "second wavefront lane");
-            partner->stream_override = partner->own_stream;
         }
         partner->cfg = cfg;
-        if ((st = partner->reserve(chunk_cap)) || (st = partner->ensure_overflow(need, partner->own_stream))) return st;
+        if ((st = partner->reserve(chunk_cap)) || (st = partner->ensure_overflow(need, partner->stream_override))) return st;
         lane[1] = partner;
     }
     // host tables: scenes, jobs, then the units of every chunk (job, block, offset)
@@ -1137,12 +1135,12 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
     const bool two = lanes == 2 && chunks.size() > 1;
     if (two) {  // the second lane starts after the tables are on the device
         if ((e = hipEventRecord(ev_start, stream)) != hipSuccess ||
-            (e = hipStreamWaitEvent(partner->own_stream, ev_start, 0)) != hipSuccess)
+            (e = hipStreamWaitEvent(partner->stream_override, ev_start, 0)) != hipSuccess)
             return hip_error(e, "second lane start");
     }
     for (size_t c = 0; c < chunks.size(); ++c) {
         Wavefront& w = *lane[two ? c % 2 : 0];
-        const hipStream_t lane_stream = &w == this ? stream : w.own_stream;
+        const hipStream_t lane_stream = &w == this ? stream : w.stream_override;
         Launch L;
         L.grid = grid;
         L.trace_grid = trace_grid;
@@ -1203,7 +1201,7 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
         }
     }
     if (two) {  // the caller's stream resumes after both lanes
-        if ((e = hipEventRecord(ev_done, partner->own_stream)) != hipSuccess ||
+        if ((e = hipEventRecord(ev_done, partner->stream_override)) != hipSuccess ||
             (e = hipStreamWaitEvent(stream, ev_done, 0)) != hipSuccess)
             return hip_error(e, "second lane join");
     }

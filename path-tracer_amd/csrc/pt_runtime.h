@@ -26,6 +26,11 @@ struct Context {
     uint64_t hbm_bytes = 0;
     char arch[64] = {0};
     hipStream_t stream = nullptr;
+    // the second stream of two-lane work: every render's second chunk lane and
+    // the render server's second round lane. A process holds few streams this
+    // way, and the runtime spreads them over its hardware queues (4 by
+    // default), so the two lanes do not share a queue and serialise.
+    hipStream_t lane_stream = nullptr;
     float* srgb_lut = nullptr;            // 256-entry sRGB decode table
     uint32_t* ovf = nullptr;              // grid-strided traversal overflow stacks
     size_t ovf_bytes = 0;

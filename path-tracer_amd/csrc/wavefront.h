@@ -118,10 +118,10 @@ struct Wavefront {
     ptd::WaveConfig cfg;
     hipStream_t stream_override = nullptr;  // run on this stream instead of the context's
     bool one_lane = false;                  // never split a render over a second lane (the caller pipelines)
-    // second chunk lane (cfg.streams == 2): its own path state, overflow
-    // stacks and stream; odd chunks run there, overlapping the even ones
+    // second chunk lane (cfg.streams == 2): its own path state and overflow
+    // stacks, on the context's lane stream; odd chunks run there, overlapping
+    // the even ones
     Wavefront* partner = nullptr;
-    hipStream_t own_stream = nullptr;       // created for a partner lane
     hipEvent_t ev_start = nullptr, ev_done = nullptr;
     pt_status ensure_overflow(size_t bytes, hipStream_t stream);
     pt_status reserve(uint32_t paths);
