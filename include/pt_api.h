@@ -54,11 +54,15 @@ const char* pt_version(void);
 /* ------------------------------------------------------------------ context */
 typedef struct pt_context pt_context;
 
-/* Select HIP device `device` and create a stream for this context.
+/* Select HIP device `device` and create the context's two streams: all work
+ * of the context runs on them (a render's second lane and a render server's
+ * second lane on the second one).
  * Replaces gpu.rs:14-76 (adapter + device request). */
 pt_status pt_context_create(int device, pt_context** out);
+/* Waits for both streams, then releases the context. Servers, tracers and
+ * scenes made on it must be destroyed first. */
 pt_status pt_context_destroy(pt_context* ctx);
-/* Wait for all work queued on the context's stream. */
+/* Wait for all work queued on the context's two streams. */
 pt_status pt_context_synchronize(pt_context* ctx);
 /* The device facts behind launch sizes and rate reports; replaces the
  * adapter.limits() / adapter.get_info() queries of gpu.rs:39-44. */
