@@ -253,13 +253,19 @@ def measure_job(ctx, name, steps, warmup, schedule_name="wavefront"):
         accesses = (4 * (counts["pair_tests"] + counts["shadow_pair_tests"]) +
                     3 * (counts["tri_tests"] + counts["shadow_tri_tests"]))
         cu_cycles = one_s * one_per_step * info["clock_khz"] * 1e3 * info["cu_count"]
+        rate = accesses / cu_cycles if cu_cycles else None
+        ceiling = ctx.measure_gather(1)
         roof["one_lane"]["l1_gather"] = {
-            "lane_accesses_per_cu_cycle": accesses / cu_cycles if cu_cycles else None,
+            "lane_accesses_per_cu_cycle": rate,
+            "ceiling": ceiling, "frac": rate / ceiling if rate and ceiling else None,
+            "ceiling_four_lanes_per_segment": ctx.measure_gather(4),
             "td_busy": load_l1(name),
             "cu_count": info["cu_count"], "clock_mhz": info["clock_khz"] / 1e3,
             "note": "lane accesses = 4 per child-pair test + 3 per triangle test (loads of at most 16 B), over "
-                    "the one-lane traversal time at the peak clock; td_busy is the L1 data-return unit's busy "
-                    "fraction from the committed rocprofv3 counters"}
+                    "the one-lane traversal time at the peak clock; ceiling = the same rate measured on this GPU "
+                    "for scattered 16-byte loads of L1-resident data, each lane its own 64-byte segment "
+                    "(pt_context_measure_gather); td_busy is the L1 data-return unit's busy fraction from the "
+                    "committed rocprofv3 counters"}
         out["roofline"] = roof
         out["kernel_ms_per_step"] = {"extend_busy": busy_s_per_step * 1e3,
                                      "extend_events": ev_all["extend_ms"] / extra_steps,

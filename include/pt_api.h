@@ -75,6 +75,15 @@ typedef struct pt_context_info {
     char arch[64];          /* e.g. "gfx950:sramecc+:xnack-" */
 } pt_context_info;
 pt_status pt_context_get_info(const pt_context* ctx, pt_context_info* info);
+/* Measures the vector L1's gather rate on the context's GPU: lane accesses
+ * per CU per cycle (at the peak clock) of 16-byte loads from an L1-resident
+ * buffer, every lane at a pseudo-random 64-byte segment (lanes_per_segment 1:
+ * the traversal's divergent child-pair loads) or four consecutive lanes at the
+ * four parts of one segment (lanes_per_segment 4). Times a probe kernel with
+ * HIP events on the context's stream and synchronises it. No counterpart in
+ * the reference: it is the measured ceiling that rate reports set the
+ * traversal's lane-access rate against. */
+pt_status pt_context_measure_gather(pt_context* ctx, uint32_t lanes_per_segment, double* lane_accesses_per_cu_cycle);
 
 /* -------------------------------------------------------------- host world */
 /* Input of the host scene preparation: a triangle soup in input order (the

@@ -281,6 +281,38 @@ pt_status pt_context_get_info(const pt_context* c, pt_context_info* info) {
     return PT_OK;
 }
 
+pt_status pt_context_measure_gather(pt_context* c, uint32_t lanes_per_segment, double* rate) {
+    if (!c || !rate) return set_error(PT_ERR_INVALID, "pt_context_measure_gather: null argument");
+    if (lanes_per_segment != 1 && lanes_per_segment != 4)
+        return set_error(PT_ERR_INVALID, "lanes_per_segment must be 1 or 4");
+    if (pt_status st = use_device(c->device)) return st;
+    const uint32_t blocks = c->cu_count * 64u;
+    DeviceBuffer data, out;
+    if (pt_status st = data.alloc(gather_probe_data_bytes())) return st;
+    if (pt_status st = out.alloc((size_t)blocks * 64u * sizeof(float))) return st;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMemsetAsync(data.ptr, 0, data.bytes, c->stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    constexpr int TIMED = 5;
+    const uint32_t shared = lanes_per_segment == 4 ? 1u : 0u;
+    if (e == hipSuccess) e = launch_gather_probe((const float4*)data.ptr, shared, (float*)out.ptr, blocks, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+    for (int i = 0; i < TIMED && e == hipSuccess; ++i)
+        e = launch_gather_probe((const float4*)data.ptr, shared, (float*)out.ptr, blocks, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return hip_error(e, "gather probe");
+    const double cycles = (double)ms * 1e-3 * (double)c->clock_khz * 1e3;
+    if (cycles <= 0.0 || c->cu_count == 0) return set_error(PT_ERR_DEVICE, "gather probe: no elapsed time");
+    *rate = (double)gather_probe_lane_accesses(blocks) * TIMED / ((double)c->cu_count * cycles);
+    return PT_OK;
+}
+
 pt_status pt_context_synchronize(pt_context* c) {
     if (!c) return set_error(PT_ERR_INVALID, "null context");
     hipError_t e = hipStreamSynchronize(c->stream);

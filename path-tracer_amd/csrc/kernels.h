@@ -57,5 +57,10 @@ hipError_t launch_occluded(const ptd::DevScene& s, const float* rays, const floa
 hipError_t launch_display(const float4* storage, uint32_t n, uchar4* out, hipStream_t stream);
 // Device address of the no-alloc scratch array on `device` (which must be current).
 hipError_t noalloc_scratch_address(int device, uint32_t** out);
+// The L1 gather-rate probe (gather_probe.hip): its data buffer size, the lane
+// accesses one launch of `blocks` one-wave blocks makes, and the launch.
+size_t gather_probe_data_bytes();
+uint64_t gather_probe_lane_accesses(uint32_t blocks);
+hipError_t launch_gather_probe(const float4* data, uint32_t shared, float* out, uint32_t blocks, hipStream_t stream);
 
 }  // namespace ptcore

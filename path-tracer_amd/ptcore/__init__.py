@@ -198,6 +198,7 @@ _SIGNATURES = [
     ("pt_context_destroy", _i32, [_p]),
     ("pt_context_synchronize", _i32, [_p]),
     ("pt_context_get_info", _i32, [_p, C.POINTER(ContextInfo)]),
+    ("pt_context_measure_gather", _i32, [_p, C.c_uint32, C.POINTER(C.c_double)]),
     ("pt_world_build", _i32, [C.POINTER(WorldDesc), C.POINTER(_p)]),
     ("pt_world_destroy", _i32, [_p]),
     ("pt_world_scene_desc", _i32, [_p, C.POINTER(SceneDesc)]),
@@ -753,6 +754,13 @@ class Context:
         _check(lib().pt_context_get_info(self.handle, C.byref(i)))
         return {"device": i.device, "cu_count": i.cu_count, "clock_khz": i.clock_khz, "l2_bytes": i.l2_bytes,
                 "hbm_bytes": i.hbm_bytes, "arch": i.arch.decode("utf-8", "replace")}
+
+    def measure_gather(self, lanes_per_segment=1):
+        """pt_context_measure_gather: the L1's rate of 16-byte lane accesses per CU per cycle, each lane its
+        own 64-byte segment (1) or four lanes per segment (4)."""
+        r = C.c_double()
+        _check(lib().pt_context_measure_gather(self.handle, lanes_per_segment, C.byref(r)))
+        return r.value
 
     def close(self):
         if self.handle:

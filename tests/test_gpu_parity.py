@@ -313,3 +313,14 @@ def test_context_info(ctx):
     print(i, flush=True)
     assert i["arch"].startswith("gfx950") and i["cu_count"] > 0 and i["clock_khz"] > 0
     assert i["hbm_bytes"] > (1 << 30) and i["l2_bytes"] > 0
+
+
+def test_measure_gather(ctx):
+    """pt_context_measure_gather gives a positive lane-access rate, a higher one when four lanes share each
+    64-byte segment than when every lane has its own, and refuses other lane counts."""
+    own = ctx.measure_gather(1)
+    shared = ctx.measure_gather(4)
+    print(f"gather: {own:.3f} lane accesses per CU per cycle (own segment), {shared:.3f} (four lanes)", flush=True)
+    assert 0.05 < own < 64.0 and shared > 1.5 * own
+    with pytest.raises(ptcore.PtError):
+        ctx.measure_gather(2)
