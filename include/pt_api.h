@@ -84,6 +84,17 @@ pt_status pt_context_get_info(const pt_context* ctx, pt_context_info* info);
  * the reference: it is the measured ceiling that rate reports set the
  * traversal's lane-access rate against. */
 pt_status pt_context_measure_gather(pt_context* ctx, uint32_t lanes_per_segment, double* lane_accesses_per_cu_cycle);
+/* Checks, on the context's GPU, the shading code's restricted-domain square
+ * root and division against the device's correctly rounded sqrtf and
+ * division. The root is compared on every float32 bit pattern from
+ * sqrt_first to sqrt_last (inclusive); the division on div_pairs pseudo-random
+ * operand pairs from its domain (divisor magnitude in [2^-60, 2^60), dividend
+ * +0 or of magnitude in [2^-60, 2^4), both signs), drawn from div_seed.
+ * Results that differ (NaN equals NaN) are counted into *sqrt_mismatches and
+ * *div_mismatches. Synchronises the context's stream. No counterpart in the
+ * reference: it pins the arithmetic that keeps the kernels bit-exact. */
+pt_status pt_context_check_fast_math(pt_context* ctx, uint32_t sqrt_first, uint32_t sqrt_last, uint64_t div_pairs,
+                                     uint32_t div_seed, uint64_t* sqrt_mismatches, uint64_t* div_mismatches);
 
 /* -------------------------------------------------------------- host world */
 /* Input of the host scene preparation: a triangle soup in input order (the

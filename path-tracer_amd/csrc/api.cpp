@@ -313,6 +313,29 @@ pt_status pt_context_measure_gather(pt_context* c, uint32_t lanes_per_segment, d
     return PT_OK;
 }
 
+pt_status pt_context_check_fast_math(pt_context* c, uint32_t sqrt_first, uint32_t sqrt_last, uint64_t div_pairs,
+                                     uint32_t div_seed, uint64_t* sqrt_mismatches, uint64_t* div_mismatches) {
+    if (!c || !sqrt_mismatches || !div_mismatches)
+        return set_error(PT_ERR_INVALID, "pt_context_check_fast_math: null argument");
+    if (sqrt_last < sqrt_first) return set_error(PT_ERR_INVALID, "pt_context_check_fast_math: sqrt_last < sqrt_first");
+    if (pt_status st = use_device(c->device)) return st;
+    DeviceBuffer counts;
+    if (pt_status st = counts.alloc(2 * sizeof(unsigned long long))) return st;
+    unsigned long long* d = (unsigned long long*)counts.ptr;
+    const uint32_t blocks = std::max(1u, c->cu_count) * 16u;
+    hipError_t e = hipMemsetAsync(d, 0, counts.bytes, c->stream);
+    if (e == hipSuccess)
+        e = launch_check_sqrt(sqrt_first, (uint64_t)sqrt_last - sqrt_first + 1u, d, blocks, c->stream);
+    if (e == hipSuccess && div_pairs) e = launch_check_div(div_pairs, div_seed, d + 1, blocks, c->stream);
+    unsigned long long h[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_error(e, "fast math check");
+    *sqrt_mismatches = h[0];
+    *div_mismatches = h[1];
+    return PT_OK;
+}
+
 pt_status pt_context_synchronize(pt_context* c) {
     if (!c) return set_error(PT_ERR_INVALID, "null context");
     hipError_t e = hipStreamSynchronize(c->stream);

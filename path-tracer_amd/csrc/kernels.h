@@ -62,5 +62,12 @@ hipError_t noalloc_scratch_address(int device, uint32_t** out);
 size_t gather_probe_data_bytes();
 uint64_t gather_probe_lane_accesses(uint32_t blocks);
 hipError_t launch_gather_probe(const float4* data, uint32_t shared, float* out, uint32_t blocks, hipStream_t stream);
+// The restricted-domain sqrt_nt_ / div_nt_ against sqrtf and division
+// (fast_math_check.hip): bit patterns lo .. lo + n - 1 (lo + n <= 2^32), and n
+// seeded operand pairs; each adds its mismatch count to *mismatches.
+hipError_t launch_check_sqrt(uint32_t lo, uint64_t n, unsigned long long* mismatches, uint32_t blocks,
+                             hipStream_t stream);
+hipError_t launch_check_div(uint64_t n, uint32_t seed, unsigned long long* mismatches, uint32_t blocks,
+                            hipStream_t stream);
 
 }  // namespace ptcore

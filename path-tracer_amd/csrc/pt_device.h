@@ -729,21 +729,28 @@ __device__ __forceinline__ f3 orthonormal_tangent(f3 tangent, f3 normal) {
     return f3{ptm::fma_(sg * normal.x * normal.x, a, 1.0f), sg * normal.x * normal.y * a, -sg * normal.x};
 }
 __device__ __forceinline__ float fifth_power(float x) { return x * x * x * x * x; }
+// ior is a modulated IOR (specular_ior_modulated: within [2^-25, 2^25] or NaN)
+// and cos_i the cosine between two unit vectors. So the first division's
+// numerator 1 - cos_i^2 is +0 or at least 2^-24 in magnitude, and its
+// denominator lies in [2^-50, 2^50]: the domain of div_nt_. Below, x < 1 makes
+// 1 - x at least 2^-24 (or NaN): the domain of sqrt_nt_.
 __device__ __forceinline__ float fresnel_exact(float ior, float cos_i) {
-    float x = ptm::fma_(-cos_i, cos_i, 1.0f) / (ior * ior);
+    float x = ptm::div_nt_(ptm::fma_(-cos_i, cos_i, 1.0f), ior * ior);
     if (x >= 1.0f) return 1.0f;
-    float cos_t = sqrtf(1.0f - x);
+    float cos_t = ptm::sqrt_nt_(1.0f - x);
     float Rs = ptm::fma_(-ior, cos_t, cos_i) / ptm::fma_(ior, cos_t, cos_i);
     float Rp = ptm::fma_(-ior, cos_i, cos_t) / ptm::fma_(ior, cos_i, cos_t);
     return 0.5f * (Rs * Rs + Rp * Rp);
 }
+// the root's argument is 1 plus a ratio of squares: at least 1, +inf or NaN
 __device__ __forceinline__ float ggx_lambda(f3 v, float ax, float ay) {
-    return ptm::fma_(0.5f, sqrtf(1.0f + (ax * ax * v.x * v.x + ay * ay * v.y * v.y) / (v.z * v.z)), -0.5f);
+    return ptm::fma_(0.5f, ptm::sqrt_nt_(1.0f + (ax * ax * v.x * v.x + ay * ay * v.y * v.y) / (v.z * v.z)), -0.5f);
 }
 __device__ __forceinline__ f3 sample_vndf(f3 wo, float ax, float ay, float r, float two_pi_r) {
     f3 wo_warped = ptm::normalize(f3{wo.x * ax, wo.y * ay, wo.z});
     float z = ptm::fma_(1.0f - r, 1.0f + wo_warped.z, -wo_warped.z);
-    float sin_theta = sqrtf(ptm::saturate_(ptm::fma_(-z, z, 1.0f)));
+    // 1 - z^2 of a float z is 0, at least 2^-24 in magnitude, or NaN; clamped to [0, 1]
+    float sin_theta = ptm::sqrt_nt_(ptm::saturate_(ptm::fma_(-z, z, 1.0f)));
     float sn, cs;
     ptm::sincos_(two_pi_r, sn, cs);
     float x = sin_theta * cs;
@@ -760,7 +767,7 @@ __device__ __forceinline__ float specular_ior_modulated(float ior, float weight)
 }
 __device__ __forceinline__ float e_fon_exact(float mu, float r) {
     float AF = 1.0f / ptm::fma_(0.28779340921f, r, 1.0f);
-    float Si = sqrtf(ptm::fma_(-mu, mu, 1.0f));
+    float Si = ptm::sqrt_nt_(ptm::fma_(-mu, mu, 1.0f));  // 1 - mu^2: 0, NaN or at least 2^-24 in magnitude
     float G = Si * (ptm::acos_(mu) - Si * mu) + ptm::bits_to_float(0x3f2aaaabu) * ((Si / mu) * (1.0f - (Si * Si * Si)) - Si);
     return AF + r * AF * ptm::INV_PI * G;
 }

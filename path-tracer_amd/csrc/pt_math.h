@@ -56,6 +56,46 @@ PT_HD uint32_t float_to_bits(float f) {
 PT_HD float fma_(float a, float b, float c) { return fmaf(a, b, c); }
 PT_HD float sqrt_(float x) { return sqrtf(x); }
 PT_HD float rsqrt_(float x) { return 1.0f / sqrtf(x); }
+
+// sqrtf for an argument that is +-0, at least 2^-96, +inf or NaN; each caller
+// states why its argument is one of these. For such arguments the correctly
+// rounded root is the hardware square root (within 1 ulp) followed by one test
+// of each neighbouring float s': x - s'*s <= 0 takes the one below, > 0 the one
+// above. These are the steps of the device's correctly rounded sqrtf without
+// its rescaling of tiny arguments and its special-value selects, which change
+// nothing for these arguments: the result has sqrtf's bits (+-0, +inf and NaN
+// come through the tests unchanged; negative normal arguments give NaN, as
+// sqrtf does). The host calls sqrtf.
+PT_HD float sqrt_nt_(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float below = bits_to_float(float_to_bits(s) - 1u);
+    const float above = bits_to_float(float_to_bits(s) + 1u);
+    const float r = fmaf(-below, s, x) <= 0.0f ? below : s;
+    return fmaf(-above, s, x) > 0.0f ? above : r;
+#else
+    return sqrtf(x);
+#endif
+}
+
+// a / b when the division needs no operand rescaling and no special-case
+// fixup: b normal with 2^-60 <= |b| <= 2^60, and a +0 (not -0), NaN or
+// 2^-60 <= |a| <= 2^4 (each caller states why). The correctly rounded quotient
+// is then the hardware reciprocal refined once by Newton's step and the
+// quotient corrected twice by its fma residual, the device division's own
+// steps without the scaling and fixup instructions that are identities on this
+// domain; the bits equal a / b. The host divides.
+PT_HD float div_nt_(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    float r = __builtin_amdgcn_rcpf(b);
+    r = fmaf(fmaf(-b, r, 1.0f), r, r);
+    float q = a * r;
+    q = fmaf(fmaf(-b, q, a), r, q);
+    return fmaf(fmaf(-b, q, a), r, q);
+#else
+    return a / b;
+#endif
+}
 // min/max with the C fmin/fmax NaN rule (return the non-NaN operand)
 PT_HD float min_(float a, float b) { return fminf(a, b); }
 PT_HD float max_(float a, float b) { return fmaxf(a, b); }

@@ -199,6 +199,8 @@ _SIGNATURES = [
     ("pt_context_synchronize", _i32, [_p]),
     ("pt_context_get_info", _i32, [_p, C.POINTER(ContextInfo)]),
     ("pt_context_measure_gather", _i32, [_p, C.c_uint32, C.POINTER(C.c_double)]),
+    ("pt_context_check_fast_math", _i32, [_p, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
+                                          C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("pt_world_build", _i32, [C.POINTER(WorldDesc), C.POINTER(_p)]),
     ("pt_world_destroy", _i32, [_p]),
     ("pt_world_scene_desc", _i32, [_p, C.POINTER(SceneDesc)]),
@@ -761,6 +763,15 @@ class Context:
         r = C.c_double()
         _check(lib().pt_context_measure_gather(self.handle, lanes_per_segment, C.byref(r)))
         return r.value
+
+    def check_fast_math(self, sqrt_first, sqrt_last, div_pairs=0, div_seed=1):
+        """pt_context_check_fast_math: mismatches of the restricted-domain square root against sqrtf over the
+        float32 bit patterns sqrt_first..sqrt_last, and of the restricted-domain division against a / b over
+        div_pairs seeded operand pairs. Returns (sqrt_mismatches, div_mismatches)."""
+        ms, md = C.c_uint64(), C.c_uint64()
+        _check(lib().pt_context_check_fast_math(self.handle, sqrt_first, sqrt_last, div_pairs, div_seed,
+                                                C.byref(ms), C.byref(md)))
+        return ms.value, md.value
 
     def close(self):
         if self.handle:

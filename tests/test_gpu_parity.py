@@ -324,3 +324,24 @@ def test_measure_gather(ctx):
     assert 0.05 < own < 64.0 and shared > 1.5 * own
     with pytest.raises(ptcore.PtError):
         ctx.measure_gather(2)
+
+
+def test_fast_math_matches_correct_rounding(ctx):
+    """The restricted-domain square root equals the device's correctly rounded sqrtf on every float32 in its
+    domain (+-0, [2^-96, +inf], NaN) and on the negative normals, -inf and negative NaNs (all NaN); the
+    restricted-domain division equals a / b on 2^28 operand pairs of its domain for each of two seeds."""
+    assert ctx.check_fast_math(0x00000000, 0x00000000) == (0, 0)  # +0
+    assert ctx.check_fast_math(0x80000000, 0x80000000) == (0, 0)  # -0
+    assert ctx.check_fast_math(0x0F800000, 0x7FFFFFFF) == (0, 0)  # 2^-96 .. +inf, positive NaNs
+    assert ctx.check_fast_math(0x80800000, 0xFFFFFFFF) == (0, 0)  # negative normals, -inf, negative NaNs
+    # outside the domain the hardware root alone does not give sqrtf's result (the compiler's sequence
+    # rescales these arguments first); no caller passes them: each caller's argument is 0, NaN, or at
+    # least 2^-24 in magnitude
+    tiny, _ = ctx.check_fast_math(0x00000001, 0x0F7FFFFF)
+    neg_sub, _ = ctx.check_fast_math(0x80000001, 0x807FFFFF)
+    print(f"sqrt outside the domain: (0, 2^-96) {tiny} of {0x0F7FFFFF} differ, negative subnormals "
+          f"{neg_sub} of {0x7FFFFF}", flush=True)
+    for seed in (1, 2):
+        assert ctx.check_fast_math(0, 0, 1 << 28, seed) == (0, 0), f"division seed {seed}"
+    with pytest.raises(ptcore.PtError):
+        ctx.check_fast_math(5, 4)
