@@ -746,6 +746,15 @@ __device__ __forceinline__ float fresnel_exact(float ior, float cos_i) {
 __device__ __forceinline__ float ggx_lambda(f3 v, float ax, float ay) {
     return ptm::fma_(0.5f, ptm::sqrt_nt_(1.0f + (ax * ax * v.x * v.x + ay * ay * v.y * v.y) / (v.z * v.z)), -0.5f);
 }
+// BOUNDED: the caller has checked wo.z >= 0 and ax, ay <= 2^56. The result's
+// normalisation then takes the restricted-domain root and division. Its
+// squared length is at least h.z^2 >= 2^-48: wo_warped.z >= 0 and r is a
+// random float at most 1 - 2^-23, so z + wo_warped.z, which is
+// (1 - r)(1 + wo_warped.z) >= 2^-23 less the rounding of z (at most 2^-25,
+// or 2^-24 when 1 + wo_warped.z > 2), is at least 3 * 2^-25. It is at most
+// 2^118, as |h.x|, |h.y| <= 2 (plus rounding) and ax, ay <= 2^56. So the root
+// lies in [2^-24, 2^59] and 1 / root in div_nt_'s domain.
+template <bool BOUNDED = false>
 __device__ __forceinline__ f3 sample_vndf(f3 wo, float ax, float ay, float r, float two_pi_r) {
     f3 wo_warped = ptm::normalize(f3{wo.x * ax, wo.y * ay, wo.z});
     float z = ptm::fma_(1.0f - r, 1.0f + wo_warped.z, -wo_warped.z);
@@ -756,7 +765,9 @@ __device__ __forceinline__ f3 sample_vndf(f3 wo, float ax, float ay, float r, fl
     float x = sin_theta * cs;
     float y = sin_theta * sn;
     f3 h = f3{x, y, z} + wo_warped;
-    return ptm::normalize(f3{h.x * ax, h.y * ay, h.z});
+    const f3 hw = f3{h.x * ax, h.y * ay, h.z};
+    if (BOUNDED) return hw * ptm::div_nt_(1.0f, ptm::sqrt_nt_(ptm::dot(hw, hw)));  // = normalize(hw)
+    return ptm::normalize(hw);
 }
 __device__ __forceinline__ float specular_ior_modulated(float ior, float weight) {
     float num = ior - 1.0f;
@@ -848,6 +859,7 @@ __device__ __forceinline__ void evaluate_lobes(const Bsdf& b, f3 wo, f3 wi, f3& 
     pdf_total = b.p_ggx * pdf_ggx + b.p_diffuse * pdf_eon;
 }
 
+template <bool BOUNDED>
 __device__ __forceinline__ f3 estimate_e_dielectric(f3 wo, float opl, float ax, float ay, float sim, f3 spec_color,
                                                     Rng& rng) {
     float sum_W = 0.0f;
@@ -855,7 +867,7 @@ __device__ __forceinline__ f3 estimate_e_dielectric(f3 wo, float opl, float ax, 
     for (uint32_t i = 0; i < 16u; ++i) {
         float rand1 = rng.next();
         float two_pi_rand = 2.0f * ptm::PI * rng.next();
-        f3 H = sample_vndf(wo, ax, ay, rand1, two_pi_rand);
+        f3 H = sample_vndf<BOUNDED>(wo, ax, ay, rand1, two_pi_rand);
         f3 wi = ptm::reflect(-wo, H);
         if (wi.z <= 0.0f) continue;
         float F = fresnel_exact(sim, ptm::dot(H, wo));
@@ -974,7 +986,11 @@ __device__ __forceinline__ void setup_surface(const DevScene& s, uint32_t hit, f
         b.bxx = wo.x * inv;
         b.bxy = wo.y * inv;
     }
-    b.E_dielectric = estimate_e_dielectric(wo, b.one_plus_lambda_wo, b.ax, b.ay, b.sim, b.specular_color, rng);
+    // sample_vndf's bounds: wo in the upper hemisphere (all but rare grazing
+    // cases) and any physical roughness; the general form keeps the bits otherwise
+    b.E_dielectric = wo.z >= 0.0f && fmaxf(b.ax, b.ay) <= 0x1p56f
+                         ? estimate_e_dielectric<true>(wo, b.one_plus_lambda_wo, b.ax, b.ay, b.sim, b.specular_color, rng)
+                         : estimate_e_dielectric<false>(wo, b.one_plus_lambda_wo, b.ax, b.ay, b.sim, b.specular_color, rng);
     const f3 LUMA = {0.2126f, 0.7152f, 0.0722f};
     float albedo_gloss = ptm::dot(b.E_dielectric, LUMA);
     float albedo_diffuse = (1.0f - albedo_gloss) * b.base_weight * ptm::dot(b.base_color, LUMA);
