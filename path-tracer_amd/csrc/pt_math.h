@@ -213,12 +213,14 @@ PT_HD float asin_poly(float z) {  // 0 <= z <= 1/2
     p = fma_(t, p, 0.1666666716337204f);
     return fma_(z * t, p, z);
 }
+// 1 - |x| of a float x is 0 or at least 2^-24 in magnitude, so both roots'
+// arguments (halved) are 0, NaN or at least 2^-25: sqrt_nt_'s domain
 PT_HD float acos_(float x) {
     if (x > 0.5f) {
-        return 2.0f * asin_poly(sqrtf((1.0f - x) * 0.5f));
+        return 2.0f * asin_poly(sqrt_nt_((1.0f - x) * 0.5f));
     }
     if (x < -0.5f) {
-        return PI_F - 2.0f * asin_poly(sqrtf((1.0f + x) * 0.5f));
+        return PI_F - 2.0f * asin_poly(sqrt_nt_((1.0f + x) * 0.5f));
     }
     float ax = fabsf(x);
     float s = asin_poly(ax);

@@ -10,11 +10,13 @@ import numpy as np
 import ptcore
 
 
-def random_openpbr_world(seed, image_w=128, image_h=128, atlas_size=16):
+def random_openpbr_world(seed, image_w=128, image_h=128, atlas_size=16, huge_roughness=False):
     """Cornell geometry whose seven surface materials are drawn at random
     (the emitter keeps its emission). Every parameter the shading reads
     varies: trace.slang:369-375 (anisotropic alpha), 1097-1181 (BSDF setup),
-    1026-1039 (normal map)."""
+    1026-1039 (normal map). huge_roughness: three surfaces get a specular
+    roughness of 3e8 (GGX alpha near 9e16, beyond the 2^56 bound under which
+    the E_dielectric estimate takes its restricted-domain normalisation)."""
     rng = np.random.default_rng(seed)
     g = ptcore.Generated(ptcore.SCENE_CORNELL, 1, image_w, image_h)
     a = g.arrays()
@@ -32,6 +34,8 @@ def random_openpbr_world(seed, image_w=128, image_h=128, atlas_size=16):
     mat["specular_roughness_anisotropy"][surf] = np.where(rng.uniform(size=n) < 0.3, pick(0.0, 1.0),
                                                           rng.uniform(0, 1, n))
     mat["specular_ior"][surf] = rng.uniform(1.1, 2.5, n)
+    if huge_roughness:
+        mat["specular_roughness"][surf[:3]] = 3.0e8
     # tangent-space normal maps on about half the surfaces
     mapped = surf[rng.uniform(size=n) < 0.6]
     mat["normal_tex_layer"][mapped] = 0

@@ -307,6 +307,26 @@ def test_random_openpbr_materials_match_oracle(ctx, seed):
     scene.close()
 
 
+def test_huge_roughness_takes_the_general_estimate(ctx):
+    """Surfaces with GGX alpha beyond 2^56 run the E_dielectric estimate in its general form (full
+    normalisation of each sampled half vector); the image still equals the oracle's under both
+    schedules, as do the other surfaces, which take the restricted-domain form."""
+    from materials import random_openpbr_world
+    world = random_openpbr_world(5, huge_roughness=True)
+    scene = ptcore.Scene(ctx, world.desc)
+    w = h = 64
+    ref, _, _ = oracle.render_f32(world.desc, 128, 128, 6, 4, tile=(w, h), origin=(32, 32))
+    assert np.isfinite(ref).all()
+    for schedule in (ptcore.SCHEDULE_MEGAKERNEL, ptcore.SCHEDULE_WAVEFRONT):
+        t = ptcore.Tracer(ctx, scene, w, h, schedule)
+        t.set_tile((128, 128), (32, 32), (w, h))
+        t.set_max_bounces(6)
+        t.trace(4)
+        assert_parity(t.accumulation(), ref, f"huge roughness schedule={schedule}")
+        t.close()
+    scene.close()
+
+
 def test_context_info(ctx):
     """pt_context_get_info reports the device the kernels were built for."""
     i = ctx.info()
