@@ -26,8 +26,10 @@ __device__ __forceinline__ void tile_pixel(uint32_t block, uint32_t tid, uint32_
     gy = (block / blocks_x) * 16u + (wave >> 1) * 8u + (lane >> 3);
 }
 
+// held to 3 waves per SIMD (168 VGPRs): at 2 the whole-path kernel loses a
+// third of its throughput
 template <bool COUNT>
-__global__ __launch_bounds__(256) void k_megakernel(DevScene s, TileArgs a, float4* __restrict__ accum,
+__global__ __launch_bounds__(256, 3) void k_megakernel(DevScene s, TileArgs a, float4* __restrict__ accum,
                                                      float4* __restrict__ storage, uint32_t* __restrict__ ovf,
                                                      unsigned long long* __restrict__ counters) {
     __shared__ uint32_t lds_stack[MEGA_LDS_DEPTH * 256];
