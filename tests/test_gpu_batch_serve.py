@@ -475,3 +475,69 @@ def test_server_arena_limits_admission(ctx, scenes):
     ref = render_alone(ctx, sc, [job])[0]
     for img in got:
         assert np.array_equal(img, ref)
+
+
+ALPHA_MASK, ALPHA_BLEND = 1, 2
+
+
+@pytest.fixture(scope="module")
+def alpha_world(ctx):
+    """The Cornell box with alpha-tested materials (a blend tall box, a mask back wall cut out and a
+    mask short box that passes its cutoff). A batch or server round that holds this scene shades
+    with the kernel instance that tests shadow rays inline (their any-hit walks draw random
+    numbers), for every scene in the round."""
+    w = ptcore.World.generate(ptcore.SCENE_CORNELL, 1, 256, 256)
+    m = w.arrays()["materials"]
+    for k, mode, alpha in ((5, ALPHA_BLEND, 0.55), (2, ALPHA_MASK, 0.3), (6, ALPHA_MASK, 0.8)):
+        m["alpha_mode"][k] = mode
+        m["base_color"][k, 3] = alpha
+    s = ptcore.Scene(ctx, w.desc)
+    yield w, s
+    s.close()
+
+
+@pytest.mark.parametrize("window", [1, 2, 4])
+def test_batch_with_an_alpha_scene_equals_lone_renders(ctx, scenes, alpha_world, window):
+    """Jobs on opaque scenes batched with a job on the alpha-tested scene, under each shade window:
+    every job equals its lone render bit for bit, and the alpha job's tile equals the oracle."""
+    aw, asc = alpha_world
+    _, sc = scenes
+    allsc = sc + [asc]
+    jobs = JOBS[:3] + [(len(sc), 72, 56, 4, 6)]
+    try:
+        ptcore.set_wave_config(ctx, shade_window=window)
+        alone = render_alone(ctx, allsc, jobs)
+        tracers = []
+        for si, w, h, spp, mb in jobs:
+            t = ptcore.Tracer(ctx, allsc[si], w, h)
+            t.set_max_bounces(mb)
+            tracers.append(t)
+        ptcore.trace_batch(tracers, [j[3] for j in jobs])
+        batch = [t.accumulation() for t in tracers]
+        for t in tracers:
+            t.close()
+    finally:
+        ptcore.set_wave_config(ctx)
+    for a, b, job in zip(alone, batch, jobs):
+        assert np.array_equal(a, b), (window, job)
+    si, w, h, spp, mb = jobs[-1]
+    ref = oracle.render_f32(aw.desc, w, h, mb, spp)[0]
+    assert np.array_equal(batch[-1][..., :3], ref[..., :3])
+
+
+def test_server_with_an_alpha_scene(ctx, scenes, alpha_world):
+    """Server rounds that mix the alpha-tested scene with opaque ones give every job its lone render."""
+    _, asc = alpha_world
+    _, sc = scenes
+    allsc = sc + [asc]
+    jobs = [(3, 64, 48, 3, 6), (1, 96, 64, 2, 5), (3, 40, 40, 5, 8), (2, 64, 64, 4, 6)]
+    srv = ptcore.Server(ctx, hbm_budget_bytes=8 << 30, max_active_jobs=8, round_paths=1 << 14)
+    try:
+        ids = [srv.submit(allsc[si], w, h, spp, mb) for si, w, h, spp, mb in jobs]
+        for jid in ids:
+            srv.wait(jid)
+        got = [srv.result(jid) for jid in ids]
+    finally:
+        srv.close()
+    for g, r, job in zip(got, render_alone(ctx, allsc, jobs), jobs):
+        assert np.array_equal(g, r), job
