@@ -1,7 +1,7 @@
 // api.cpp - the C ABI of libptcore.so: contexts, device scenes, tracers.
 //
 // Mirrors the reference's host objects around its trace kernel:
-//   pt_context  <- wgpu adapter/device/queue (gpu.rs:14-76): one HIP device + stream
+//   pt_context  <- wgpu adapter/device/queue (gpu.rs:14-76): one HIP device + two streams
 //   pt_scene    <- WorldResources (world_resources.rs:74-245, 479-525, 600-661)
 //   pt_tracer   <- PathTracer (path_tracer.rs:59-358)
 // Limits the reference enforces with anyhow errors (tile size, buffer sizes,
