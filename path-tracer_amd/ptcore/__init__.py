@@ -418,6 +418,9 @@ class World:
         self._keep = keep
         self.desc = SceneDesc()
         _check(lib().pt_world_scene_desc(h, C.byref(self.desc)))
+        # the descriptor points into this world's arrays: it keeps the world alive,
+        # so World.generate(...).desc stays valid after the World object is dropped
+        self.desc._owner = self
 
     @classmethod
     def generate(cls, kind, seed, width, height, triangles=0):

@@ -6,6 +6,7 @@ The BVH build runs on several threads for large scenes. PT_HOST_THREADS and
 PT_BVH_TASK_GRAIN force that path onto small scenes here, so the oracle
 comparison covers both the one-thread and the threaded builds."""
 import contextlib
+import ctypes
 import os
 import subprocess
 import sys
@@ -237,3 +238,24 @@ def test_bvh_without_progress_is_an_error():
     for case in ("huge", "nan"):
         assert lines[case].startswith(f"error {ptcore.PT_ERR_INVALID} "), lines
         assert "made no progress" in lines[case]
+
+
+def test_scene_descriptor_keeps_its_world_alive():
+    """World.generate(...).desc stays valid after the World object itself is dropped: the descriptor
+    points into the world's arrays and holds a reference to it (the world is freed with the
+    descriptor)."""
+    import gc
+    import weakref
+    w = ptcore.World.generate(ptcore.SCENE_CORNELL, 1, 64, 64)
+    ref = weakref.ref(w)
+    first = np.ctypeslib.as_array(ctypes.cast(w.desc.geometries, ctypes.POINTER(ctypes.c_float)),
+                                  (w.desc.triangle_count * 9,)).copy()
+    d = w.desc
+    del w
+    gc.collect()
+    assert ref() is not None
+    again = np.ctypeslib.as_array(ctypes.cast(d.geometries, ctypes.POINTER(ctypes.c_float)), (d.triangle_count * 9,))
+    assert np.array_equal(again, first)
+    del d, again
+    gc.collect()
+    assert ref() is None
