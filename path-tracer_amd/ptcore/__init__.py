@@ -330,6 +330,7 @@ class Generated:
         self.handle = h
         self.desc = WorldDesc()
         _check(lib().pt_generated_world_desc(h, C.byref(self.desc)))
+        self.desc._owner = self  # the descriptor borrows this object's arrays
 
     def close(self):
         if self.handle:
@@ -395,6 +396,7 @@ class SceneFile:
         self.handle = h
         self.desc = SceneDesc()
         _check(lib().pt_scene_file_desc(h, C.byref(self.desc)))
+        self.desc._owner = self  # the descriptor borrows this object's arrays
 
     def close(self):
         if self.handle:
@@ -628,6 +630,7 @@ class Assets:
     def world_desc(self, skybox_luminance=-1.0, camera_position=None, yaw=None, pitch=None, fov=None, aspect=1.0):
         d = WorldDesc()
         _check(lib().pt_assets_world_desc(self.handle, C.byref(d)))
+        d._owner = self  # the descriptor borrows the loaded arrays
         d.skybox_luminance = skybox_luminance
         if camera_position is not None:
             for k in range(3):
@@ -704,6 +707,7 @@ class WorldInput:
         d.camera_pitch_degrees = nan if pitch is None else pitch
         d.camera_fov_degrees = nan if fov is None else fov
         d.aspect_ratio = aspect
+        d._owner = self  # the descriptor points into this object's arrays
         self.desc = d
 
     def build(self):
@@ -740,6 +744,7 @@ class SceneArrays:
         d.skybox_alias, d.skybox_alias_count = _ptr(a["skybox_alias"]), len(a["skybox_alias"])
         d.scene_uniforms = scene_uniforms
         d.scene_alias, d.scene_alias_count = _ptr(a["scene_alias"]), len(a["scene_alias"])
+        d._owner = self  # the descriptor points into this object's arrays
         self.desc = d
 
 
