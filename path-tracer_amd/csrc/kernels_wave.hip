@@ -449,9 +449,13 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_fetch(DevScene s0, const 
 // (the any-hit draws random numbers between NEE and BSDF sampling); only the
 // INLINE_SHADOW instance carries that traversal and its LDS stack, so launches
 // over opaque scenes keep the smaller register and LDS footprint.
-// minimum waves per SIMD the shade kernel is compiled for (its register budget)
+// minimum waves per SIMD the shade kernel is compiled for (its register
+// budget): 3 fits every instance in 168 VGPRs. The opaque instances need no
+// more; the multi-scene inline-shadow ones would otherwise take 169-181 and
+// run at 2 waves (a batch holding an alpha-tested scene: 1.25 -> 1.36 G path
+// samples/s at 3, with a few bytes of scratch).
 #ifndef PT_SHADE_WAVES_PER_EU
-#define PT_SHADE_WAVES_PER_EU 1
+#define PT_SHADE_WAVES_PER_EU 3
 #endif
 template <bool MULTI, bool INLINE_SHADOW, uint32_t WIN>
 __global__ __launch_bounds__(WAVE_BLOCK, PT_SHADE_WAVES_PER_EU) void k_shade(DevScene s0, const DevScene* __restrict__ scenes,
