@@ -644,6 +644,8 @@ __global__ __launch_bounds__(WAVE_BLOCK) void k_accumulate(WaveJob j0, const Wav
         float4 prev = jb.first_sample > 1u ? jb.accum[idx] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         f3 acc = f3{prev.x, prev.y, prev.z};
         const uint32_t base = units.offset[u] + lane;
+        // unrolled so that several samples' loads are in flight; the sum stays in sample order
+#pragma unroll 8
         for (uint32_t k = 0; k < jb.samples; ++k) acc = acc + ld3(ps.lr, ps.lg, ps.lb, base + k * 64u);
         jb.accum[idx] = make_float4(acc.x, acc.y, acc.z, 1.0f);
         if (jb.storage) {
