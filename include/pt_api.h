@@ -371,7 +371,9 @@ typedef struct pt_wave_config {
     uint32_t trace_waves_per_cu;  /* traversal grid: one-wave blocks per CU, 16..2048 (default 512) */
     uint32_t shade_blocks_per_cu; /* shade grid: 256-thread blocks per CU, 1..256 (default 32) */
     uint32_t shade_window;        /* queue rounds of 256 entries that a shade block splits into
-                                     surface hits and misses before shading: 1, 2 or 4 (default 2) */
+                                     surface hits and misses before shading: 1, 2 or 4; 0: automatic,
+                                     4 for a launch over several jobs (batch, server round), 2 for one
+                                     (default 0) */
     uint32_t lane_split_paths;    /* automatic lanes: a render of at least this many paths that fits
                                      two chunks is cut into two chunks on two lanes (default 2^21) */
 } pt_wave_config;

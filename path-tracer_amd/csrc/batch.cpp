@@ -42,8 +42,8 @@ pt_status check_config(const pt_wave_config* c) {
         return set_error(PT_ERR_INVALID, "trace_waves_per_cu must be in [16, 2048]");
     if (c->shade_blocks_per_cu < 1 || c->shade_blocks_per_cu > 256)
         return set_error(PT_ERR_INVALID, "shade_blocks_per_cu must be in [1, 256]");
-    if (c->shade_window != 1 && c->shade_window != 2 && c->shade_window != 4)
-        return set_error(PT_ERR_INVALID, "shade_window must be 1, 2 or 4");
+    if (c->shade_window != 0 && c->shade_window != 1 && c->shade_window != 2 && c->shade_window != 4)
+        return set_error(PT_ERR_INVALID, "shade_window must be 0 (automatic), 1, 2 or 4");
     if (c->lane_split_paths == 0) return set_error(PT_ERR_INVALID, "lane_split_paths must be positive");
     return PT_OK;
 }

@@ -37,7 +37,8 @@ struct WaveConfig {
     uint32_t streams = 0;             // chunk lanes running concurrently: 1, 2, or 0 = automatic
     uint32_t trace_waves_per_cu = 512;    // traversal grid: one-wave blocks per CU
     uint32_t shade_blocks_per_cu = 32;    // shade grid: 256-thread blocks per CU
-    uint32_t shade_window = 2;            // queue rounds a shade block reorders into hits and misses: 1, 2 or 4
+    uint32_t shade_window = 0;            // queue rounds a shade block reorders into hits and misses: 1, 2 or 4;
+                                          // 0: 4 for a launch over several jobs, 2 for one
     uint32_t lane_split_paths = 1u << 21; // smallest one- or two-chunk render that two lanes share
 };
 enum Traversal : uint32_t { TRAVERSAL_BATCH = 0, TRAVERSAL_REFILL = 1, TRAVERSAL_FETCH = 2 };

@@ -834,7 +834,9 @@ void run_chunk(Wavefront& w, const Launch& L, uint32_t max_bounces, bool countin
 #define PT_SHADE_LAUNCH(A, W)                                                                                   \
     k_shade<MULTI, A, W><<<L.shade_grid, WAVE_BLOCK, 0, stream>>>(*L.s0, L.scenes, L.jobs, *L.j0, ps, b, cur, cnt, nxt, \
                                                                   w.counts + 2 * (b + 1), shadow, cnt + 1, w.ovf)
-            switch ((L.any_alpha ? 8u : 0u) | w.cfg.shade_window) {
+            // automatic window: 4 rounds for a launch over several jobs, 2 for one
+            const uint32_t win = w.cfg.shade_window ? w.cfg.shade_window : (MULTI ? 4u : 2u);
+            switch ((L.any_alpha ? 8u : 0u) | win) {
                 case 1: PT_SHADE_LAUNCH(false, 1); break;
                 case 4: PT_SHADE_LAUNCH(false, 4); break;
                 case 8 | 1: PT_SHADE_LAUNCH(true, 1); break;

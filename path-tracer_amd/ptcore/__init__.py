@@ -946,7 +946,7 @@ class Tracer:
 
 
 DEFAULT_WAVE_CONFIG = dict(lds_stack_depth=16, chunk_paths=1 << 24, sort_key=SORT_NONE, traversal=0, streams=0,
-                           trace_waves_per_cu=512, shade_blocks_per_cu=32, shade_window=2, lane_split_paths=1 << 21)
+                           trace_waves_per_cu=512, shade_blocks_per_cu=32, shade_window=0, lane_split_paths=1 << 21)
 
 
 def set_wave_config(ctx: Context, **overrides):
