@@ -365,3 +365,55 @@ def test_fast_math_matches_correct_rounding(ctx):
         assert ctx.check_fast_math(0, 0, 1 << 28, seed) == (0, 0), f"division seed {seed}"
     with pytest.raises(ptcore.PtError):
         ctx.check_fast_math(5, 4)
+
+
+def test_axis_aligned_rays_match_oracle(ctx, field, field_scene):
+    """Rays with zero direction components, whose reciprocals are infinite: the slab test then
+    meets 0 * inf = NaN where an origin lies on a box plane, and the min/max rules decide. Along the
+    six axes (signed zeros included), in the coordinate planes, and from origins placed on the
+    Cornell box's axis-aligned walls and the field scene's node bounds: hit ids, distances,
+    barycentrics, per-ray counters and occlusion equal the oracle's."""
+    rng = np.random.default_rng(11)
+    cornell = ptcore.World.generate(ptcore.SCENE_CORNELL, 1, 64, 64)
+    cornell_scene = ptcore.Scene(ctx, cornell.desc)
+    axes = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+                     [-0.0, 0, 1], [0, -0.0, -1], [1, -0.0, -0.0]], dtype=np.float32)
+    try:
+        for world, scene, lo, hi in ((field, field_scene, -1.2, 1.2), (cornell, cornell_scene, -1.0, 1.0)):
+            n = 60_000
+            o = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+            d = np.empty((n, 3), np.float32)
+            k = n // 3
+            d[:k] = axes[rng.integers(0, len(axes), k)]
+            plane = rng.normal(size=(k, 3)).astype(np.float32)
+            plane[np.arange(k), rng.integers(0, 3, k)] = 0.0
+            d[k:2 * k] = plane / np.linalg.norm(plane, axis=1, keepdims=True)
+            # nearly axis-aligned: tiny components, whose reciprocals are huge but finite
+            near = axes[rng.integers(0, 6, n - 2 * k)].copy()
+            near += (rng.choice([-1.0, 1.0], (n - 2 * k, 3)) * 10.0 ** rng.uniform(-30, -8, (n - 2 * k, 3))).astype(np.float32)
+            d[2 * k:] = near
+            # a third of the origins on planes the boxes use: the scene's extreme coordinates
+            bv = world.arrays()["bvh_nodes"]
+            planes = np.concatenate([bv["aabb_min"].ravel(), bv["aabb_max"].ravel()]).astype(np.float32)
+            on = rng.uniform(size=n) < 0.33
+            axis = rng.integers(0, 3, n)
+            o[on, axis[on]] = rng.choice(planes, on.sum())
+            # and, for half the rays with zero direction components, a zero origin coordinate there:
+            # -o/d is then NaN too, the axis drops out of every slab test, and the walk goes deep
+            zero = (d == 0.0) & (rng.uniform(size=(n, 1)) < 0.5)
+            o[zero] = 0.0
+            rays = np.concatenate([o, d], axis=1).astype(np.float32)
+            hit, t, v, w, pairs, tris = scene.intersect(rays)
+            rh, rt, rv, rw, rp, rtr = oracle.intersect(world.desc, rays)
+            assert np.array_equal(hit, rh)
+            m = hit >= 0
+            # exactly axis- or plane-parallel rays miss unless their origin's coordinate on that axis
+            # is 0: a box plane's fma(lo, 1/d, -o/d) is then either inf - inf = NaN, which the min/max
+            # rules drop, or an infinity that empties the interval (the reference's own behaviour)
+            assert m[:2 * k].any() and m[2 * k:].any() and (pairs[:2 * k] > 1).any()
+            assert np.array_equal(t[m], rt[m]) and np.array_equal(v[m], rv[m]) and np.array_equal(w[m], rw[m])
+            assert np.array_equal(pairs, rp) and np.array_equal(tris, rtr)
+            tmax = np.where(m, rt * 0.999, np.float32(3.0e38)).astype(np.float32)
+            assert np.array_equal(scene.occluded(rays, tmax), oracle.occluded(world.desc, rays, tmax))
+    finally:
+        cornell_scene.close()
