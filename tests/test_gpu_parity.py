@@ -399,7 +399,7 @@ def test_axis_aligned_rays_match_oracle(ctx, field, field_scene):
             axis = rng.integers(0, 3, n)
             o[on, axis[on]] = rng.choice(planes, on.sum())
             # and, for half the rays with zero direction components, a zero origin coordinate there:
-            # -o/d is then NaN too, the axis drops out of every slab test, and the walk goes deep
+            # -o * (1/d) is then NaN too, the axis drops out of every slab test, and the walk goes deep
             zero = (d == 0.0) & (rng.uniform(size=(n, 1)) < 0.5)
             o[zero] = 0.0
             rays = np.concatenate([o, d], axis=1).astype(np.float32)
@@ -408,7 +408,7 @@ def test_axis_aligned_rays_match_oracle(ctx, field, field_scene):
             assert np.array_equal(hit, rh)
             m = hit >= 0
             # exactly axis- or plane-parallel rays miss unless their origin's coordinate on that axis
-            # is 0: a box plane's fma(lo, 1/d, -o/d) is then either inf - inf = NaN, which the min/max
+            # is 0: a box plane's fma(lo, 1/d, -o * (1/d)) is then either inf - inf = NaN, which the min/max
             # rules drop, or an infinity that empties the interval (the reference's own behaviour)
             assert m[:2 * k].any() and m[2 * k:].any() and (pairs[:2 * k] > 1).any()
             assert np.array_equal(t[m], rt[m]) and np.array_equal(v[m], rv[m]) and np.array_equal(w[m], rw[m])
