@@ -358,7 +358,9 @@ pt_status pt_tracer_stats(pt_tracer* tracer, pt_trace_stats* out);
 /* Wavefront tuning of a context, used by every wavefront render on it. */
 typedef struct pt_wave_config {
     uint32_t lds_stack_depth;  /* traversal stack entries kept in LDS: 16, 24 or 32 */
-    uint32_t chunk_paths;      /* paths in flight per chunk (the queue tile size) */
+    uint32_t chunk_paths;      /* paths in flight per chunk (the queue tile size), 4096..2^27;
+                                  0: automatic (default), 2^24, or 2^26 for a launch of at least
+                                  2^28 paths while 96 GiB of device memory are free */
     uint32_t sort_key;         /* ray-queue order: 0 none, 1 direction octant, 2 hit material */
     uint32_t traversal;        /* 0: each thread traces its rays one batch of the wave at a time;
                                   1: lane refill (a lane that finishes starts its next ray at once);
@@ -432,7 +434,7 @@ typedef struct pt_server_config {
                                    worker's path state while another job runs (a job that fits alone
                                    always runs) */
     uint32_t max_active_jobs;   /* job slots */
-    uint32_t round_paths;       /* paths per round (0: the context's chunk size) */
+    uint32_t round_paths;       /* paths per round (0: the context's chunk size; 2^24 when that is automatic) */
     uint64_t job_arena_bytes;   /* device arena for job buffers, reserved at creation
                                    (0: min(hbm_budget_bytes, 4 GiB)); a request whose buffer
                                    exceeds it is rejected */

@@ -33,8 +33,8 @@ double now_s() {
 pt_status check_config(const pt_wave_config* c) {
     if (c->lds_stack_depth != 16 && c->lds_stack_depth != 24 && c->lds_stack_depth != 32)
         return set_error(PT_ERR_INVALID, "lds_stack_depth must be 16, 24 or 32");
-    if (c->chunk_paths < 4096 || c->chunk_paths > (1u << 27))
-        return set_error(PT_ERR_INVALID, "chunk_paths must be in [4096, 2^27]");
+    if (c->chunk_paths != 0 && (c->chunk_paths < 4096 || c->chunk_paths > (1u << 27)))
+        return set_error(PT_ERR_INVALID, "chunk_paths must be 0 (automatic) or in [4096, 2^27]");
     if (c->sort_key > PT_SORT_MATERIAL) return set_error(PT_ERR_INVALID, "unknown sort key %u", c->sort_key);
     if (c->traversal > 2) return set_error(PT_ERR_INVALID, "unknown traversal mode %u", c->traversal);
     if (c->streams > 2) return set_error(PT_ERR_INVALID, "streams must be 0 (automatic), 1 or 2");
@@ -165,7 +165,7 @@ pt_status pt_autotune_batch(pt_tracer* const* tracers, uint32_t count, const uin
     };
     const Knob knobs[] = {
         {&ptd::WaveConfig::traversal, {ptd::TRAVERSAL_BATCH, ptd::TRAVERSAL_REFILL, ptd::TRAVERSAL_FETCH}},
-        {&ptd::WaveConfig::chunk_paths, {1u << 21, 1u << 22, 1u << 23, 1u << 24}},
+        {&ptd::WaveConfig::chunk_paths, {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 26}},
         {&ptd::WaveConfig::sort_key, {ptd::SORT_NONE, ptd::SORT_OCTANT, ptd::SORT_MATERIAL}},
         {&ptd::WaveConfig::lds_depth, {16, 24, 32}},
         {&ptd::WaveConfig::streams, {0, 1, 2}},
@@ -430,7 +430,8 @@ struct pt_server {
     // blocks from the active jobs in admission order, up to the round size; under mu
     Round build_round(int lane, std::vector<HostJob>& round) {
         Round r{lane, {}, 0, PT_OK};
-        const uint32_t round_paths = cfg.round_paths ? cfg.round_paths : ctx->wave_cfg.chunk_paths;
+        const uint32_t round_paths =
+            cfg.round_paths ? cfg.round_paths : ctx->wave_cfg.chunk_paths ? ctx->wave_cfg.chunk_paths : ptd::CHUNK_DEFAULT;
         for (uint64_t id : active) {
             ServerJob& j = jobs.at(id);
             if (j.cancel || j.failed || j.next_block >= j.blocks) continue;

@@ -1016,7 +1016,15 @@ pt_status Wavefront::render(const std::vector<HostJob>& jobs, int profiling, Eve
         const uint32_t be = h.block_end ? h.block_end : tile_blocks(h.job);
         total_paths += (uint64_t)(be - h.block_begin) * 64u * h.job.samples;
     }
-    uint32_t chunk_cap = std::max(cfg.chunk_paths, max_unit_paths);
+    uint32_t chunk = cfg.chunk_paths;
+    if (chunk == 0) {
+        chunk = CHUNK_DEFAULT;
+        size_t free_bytes = 0, total_bytes = 0;
+        if (total_paths >= CHUNK_LARGE_MIN_PATHS && hipMemGetInfo(&free_bytes, &total_bytes) == hipSuccess &&
+            free_bytes >= CHUNK_LARGE_MIN_FREE)
+            chunk = CHUNK_LARGE;
+    }
+    uint32_t chunk_cap = std::max(chunk, max_unit_paths);
     // lanes: a chunk's last, small bounces run beside another chunk's first,
     // large ones, and shading beside traversal. Two lanes on request, or
     // automatically for a render of three chunks or more, or of at least

@@ -945,7 +945,7 @@ class Tracer:
             pass
 
 
-DEFAULT_WAVE_CONFIG = dict(lds_stack_depth=16, chunk_paths=1 << 24, sort_key=SORT_NONE, traversal=0, streams=0,
+DEFAULT_WAVE_CONFIG = dict(lds_stack_depth=16, chunk_paths=0, sort_key=SORT_NONE, traversal=0, streams=0,
                            trace_waves_per_cu=512, shade_blocks_per_cu=32, shade_window=0, lane_split_paths=1 << 21)
 
 
