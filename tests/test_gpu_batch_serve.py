@@ -87,6 +87,34 @@ def test_wave_configs_are_bit_identical(ctx, scenes, cfg):
         assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
+def test_automatic_large_chunks_equal_small_chunks(ctx, scenes):
+    """A launch of 2^28 paths (2048x2048 at 64 spp) takes the automatic chunk
+    size, which on a device with 96 GiB or more free is 2^26 paths on two
+    lanes; the same render at explicit 2^24-path chunks must be identical."""
+    _, sc = scenes
+    w = h = 2048
+    spp = 64
+    assert w * h * spp == 1 << 28
+
+    def render():
+        t = ptcore.Tracer(ctx, sc[0], w, h)
+        t.set_max_bounces(2)
+        t.trace_all(spp)
+        assert t.sample_count == spp
+        out = t.accumulation()
+        t.close()
+        return out
+
+    auto = render()
+    try:
+        ptcore.set_wave_config(ctx, chunk_paths=1 << 24)
+        small = render()
+    finally:
+        ptcore.set_wave_config(ctx)
+    assert np.isfinite(auto).all() and auto[..., :3].max() > 0.0
+    assert np.array_equal(auto, small)
+
+
 def test_batch_matches_oracle(ctx, scenes):
     worlds, sc = scenes
     si, w, h, spp, mb = JOBS[1]
