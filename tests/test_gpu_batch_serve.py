@@ -115,6 +115,26 @@ def test_automatic_large_chunks_equal_small_chunks(ctx, scenes):
     assert np.array_equal(auto, small)
 
 
+def test_batch_of_2p28_paths_equals_lone_renders(ctx, scenes):
+    """Two jobs of 2^27 paths each: alone, each launch is below the large-chunk
+    threshold (2^24-path chunks); batched, the one launch of 2^28 paths over
+    two scenes takes the automatic large chunks. The results must not differ."""
+    _, sc = scenes
+    jobs = [(0, 2048, 1024, 64, 2), (1, 1024, 2048, 64, 2)]
+    assert sum(w * h * spp for _, w, h, spp, _ in jobs) == 1 << 28
+    alone = render_alone(ctx, sc, jobs)
+    tracers = []
+    for si, w, h, spp, mb in jobs:
+        t = ptcore.Tracer(ctx, sc[si], w, h)
+        t.set_max_bounces(mb)
+        tracers.append(t)
+    ptcore.trace_batch(tracers, [j[3] for j in jobs])
+    for t, ref, job in zip(tracers, alone, jobs):
+        assert t.sample_count == job[3]
+        assert np.array_equal(t.accumulation(), ref), job
+        t.close()
+
+
 def test_batch_matches_oracle(ctx, scenes):
     worlds, sc = scenes
     si, w, h, spp, mb = JOBS[1]
